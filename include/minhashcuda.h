@@ -1,0 +1,143 @@
+/*
+ * Public C ABI of the MI355X Weighted MinHash engine (libMHCUDA.so).
+ *
+ * The first block is the drop-in boundary: the same six entry points, the same
+ * result codes and the same parameter struct as the reference's minhashcuda.h,
+ * with the same argument meaning and error behaviour.  Each declaration cites
+ * the reference interface it replaces.  Unlike the reference header this one
+ * also compiles as C (guards added, `enum` spelled out through a typedef).
+ *
+ * The second block adds entry points the reference does not have: hashing of
+ * device-resident CSR data on a caller stream, batched hashing of many
+ * matrices in one persistent launch, kernel selection, and a serving loop.
+ */
+#ifndef MHCUDA_MINHASHCUDA_H
+#define MHCUDA_MINHASHCUDA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifdef __GNUC__
+#define MALLOC __attribute__((malloc))
+#else
+#define MALLOC
+#endif
+
+/* Opaque generator: parameter tables resident in HBM plus grow-only device
+ * buffers.  Reference: minhashcuda.h:41, minhashcuda.cc:13-38. */
+typedef struct MinhashCudaGenerator_ MinhashCudaGenerator;
+
+/* Reference: minhashcuda.h:44-48 (12-byte layout: dim@0, samples@4, verbosity@8). */
+typedef struct {
+  uint32_t dim;
+  uint16_t samples;
+  int verbosity;
+} MinhashCudaGeneratorParameters;
+
+/* Reference: minhashcuda.h:51-58, same values. */
+enum MHCUDAResult {
+  mhcudaSuccess,
+  mhcudaInvalidArguments,
+  mhcudaNoSuchDevice,
+  mhcudaMemoryAllocationFailure,
+  mhcudaRuntimeError,
+  mhcudaMemoryCopyError
+};
+typedef enum MHCUDAResult MHCUDAResult;
+
+/* Creates a generator for `dim`-dimensional rows and `samples` hashes per row.
+ * Unless `deferred`, the parameter tables are generated on the device with
+ * Philox4x32-10 from `seed` (rs ~ Gamma(2,1), ln_cs = ln Gamma(2,1),
+ * betas ~ U[0,1)).  `devices` is a bitmask (0 = all); devices that are not
+ * gfx950 are dropped and the first remaining device is used.  On error
+ * returns NULL and stores the code in *status (if non-NULL); on success
+ * *status is left untouched.
+ * Reference: minhashcuda.h:74-76, minhashcuda.cc:212-242. */
+MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
+                                  int deferred, uint32_t devices, int verbosity,
+                                  MHCUDAResult *status) MALLOC;
+
+/* {dim, samples, verbosity}; all zero for NULL.
+ * Reference: minhashcuda.h:81-82, minhashcuda.cc:244-252. */
+MinhashCudaGeneratorParameters mhcuda_get_parameters(const MinhashCudaGenerator *gen);
+
+/* Copies the three [samples][dim] float32 tables to host memory.
+ * Reference: minhashcuda.h:91-92, minhashcuda.cc:254-270. */
+MHCUDAResult mhcuda_retrieve_random_vars(const MinhashCudaGenerator *gen,
+                                         float *rs, float *ln_cs, float *betas);
+
+/* Uploads three [samples][dim] float32 tables (host memory).
+ * Reference: minhashcuda.h:101-103, minhashcuda.cc:272-285. */
+MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *gen,
+                                       const float *rs, const float *ln_cs,
+                                       const float *betas);
+
+/* Hashes `length` CSR rows held in host memory.  rows has length + 1 entries
+ * that index weights/cols absolutely (rows[0] need not be 0).  output receives
+ * length * samples * 2 uint32: (column, (uint32)(int32)t) per sample.  Returns
+ * after the output is in host memory.  An odd sample count is rejected with
+ * mhcudaInvalidArguments, as in the reference.
+ * Reference: minhashcuda.h:115-118, minhashcuda.cc:583-642. */
+MHCUDAResult mhcuda_calc(const MinhashCudaGenerator *gen, const float *weights,
+                         const uint32_t *cols, const uint32_t *rows,
+                         uint32_t length, uint32_t *output);
+
+/* Frees the generator and every device buffer; NULL is accepted.
+ * Reference: minhashcuda.h:124, minhashcuda.cc:644-649. */
+MHCUDAResult mhcuda_fini(MinhashCudaGenerator *gen);
+
+/* ------------------------------------------------------------------------ */
+/* Extensions (no counterpart in the reference).                            */
+
+/* One CSR matrix for the batched and device entry points.  All pointers are
+ * device pointers for the *_device calls and host pointers for
+ * mhcuda_calc_batch. */
+typedef struct {
+  const float *weights;
+  const uint32_t *cols;
+  const uint32_t *rows;   /* length + 1 entries, absolute positions */
+  uint32_t length;
+  uint32_t *output;       /* length * samples * 2 */
+} MHCUDAMatrix;
+
+/* Kernel choice for the hashing calls. */
+enum MHCUDAKernel {
+  mhcudaKernelAuto = 0,        /* autotuned choice */
+  mhcudaKernelPlain = 1,       /* one launch per matrix, one wavefront per row */
+  mhcudaKernelPersistent = 2   /* nnz-balanced persistent work queue */
+};
+
+/* Hashes device-resident CSR data.  `stream` is a hipStream_t; NULL means the
+ * generator's own stream and the call returns after completion, otherwise the
+ * work is only enqueued on `stream`. */
+MHCUDAResult mhcuda_calc_device(const MinhashCudaGenerator *gen,
+                                const float *weights, const uint32_t *cols,
+                                const uint32_t *rows, uint32_t length,
+                                uint32_t *output, void *stream);
+
+/* Hashes `count` device-resident matrices with the chosen kernel (plain: one
+ * launch per matrix; persistent: one launch for the whole batch). */
+MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *gen,
+                                      uint32_t count, const MHCUDAMatrix *mats,
+                                      int kernel, void *stream);
+
+/* Host-memory version of mhcuda_calc_batch_device (uploads, hashes, downloads). */
+MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *gen, uint32_t count,
+                               const MHCUDAMatrix *mats, int kernel);
+
+/* Tuning knobs of the generator (samples per lane 1/2/4, nonzeros per
+ * persistent work item, persistent waves per CU); 0 keeps the current value. */
+MHCUDAResult mhcuda_set_tuning(MinhashCudaGenerator *gen, int samples_per_lane,
+                               int segment, int waves_per_cu);
+MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen,
+                               int *samples_per_lane, int *segment,
+                               int *waves_per_cu);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* MHCUDA_MINHASHCUDA_H */

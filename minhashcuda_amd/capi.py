@@ -1,0 +1,74 @@
+"""ctypes prototypes of the C ABI in include/minhashcuda.h.
+
+Gives Python code (tests, benchmarks, users of the extension entry points)
+typed access to every function libMHCUDA.so exports, with no torch types in
+any signature.
+"""
+import ctypes
+
+RESULT_NAMES = {
+    0: "mhcudaSuccess",
+    1: "mhcudaInvalidArguments",
+    2: "mhcudaNoSuchDevice",
+    3: "mhcudaMemoryAllocationFailure",
+    4: "mhcudaRuntimeError",
+    5: "mhcudaMemoryCopyError",
+}
+
+KERNEL_AUTO, KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1, 2
+
+# Every function the public header declares, in header order.
+EXPORTS = (
+    "mhcuda_init", "mhcuda_get_parameters", "mhcuda_retrieve_random_vars",
+    "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
+    "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
+    "mhcuda_set_tuning", "mhcuda_get_tuning",
+)
+
+
+class GeneratorParameters(ctypes.Structure):
+    """MinhashCudaGeneratorParameters (minhashcuda.h)."""
+    _fields_ = [("dim", ctypes.c_uint32), ("samples", ctypes.c_uint16),
+                ("verbosity", ctypes.c_int)]
+
+
+class Matrix(ctypes.Structure):
+    """MHCUDAMatrix (minhashcuda.h)."""
+    _fields_ = [("weights", ctypes.c_void_p), ("cols", ctypes.c_void_p),
+                ("rows", ctypes.c_void_p), ("length", ctypes.c_uint32),
+                ("output", ctypes.c_void_p)]
+
+
+def load(path):
+    """Loads libMHCUDA.so and declares the prototypes of every export."""
+    L = ctypes.CDLL(path)
+    vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    L.mhcuda_init.restype = vp
+    L.mhcuda_init.argtypes = [u32, ctypes.c_uint16, u32, i32, u32, i32, ctypes.POINTER(i32)]
+    L.mhcuda_get_parameters.restype = GeneratorParameters
+    L.mhcuda_get_parameters.argtypes = [vp]
+    for name in ("mhcuda_retrieve_random_vars", "mhcuda_assign_random_vars"):
+        getattr(L, name).restype = i32
+        getattr(L, name).argtypes = [vp, vp, vp, vp]
+    L.mhcuda_calc.restype = i32
+    L.mhcuda_calc.argtypes = [vp, vp, vp, vp, u32, vp]
+    L.mhcuda_fini.restype = i32
+    L.mhcuda_fini.argtypes = [vp]
+    L.mhcuda_calc_device.restype = i32
+    L.mhcuda_calc_device.argtypes = [vp, vp, vp, vp, u32, vp, vp]
+    L.mhcuda_calc_batch_device.restype = i32
+    L.mhcuda_calc_batch_device.argtypes = [vp, u32, ctypes.POINTER(Matrix), i32, vp]
+    L.mhcuda_calc_batch.restype = i32
+    L.mhcuda_calc_batch.argtypes = [vp, u32, ctypes.POINTER(Matrix), i32]
+    L.mhcuda_set_tuning.restype = i32
+    L.mhcuda_set_tuning.argtypes = [vp, i32, i32, i32]
+    L.mhcuda_get_tuning.restype = i32
+    L.mhcuda_get_tuning.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                    ctypes.POINTER(i32)]
+    return L
+
+
+def check(code, what="call"):
+    """Raises RuntimeError unless `code` is mhcudaSuccess."""
+    if code != 0:
+        raise RuntimeError("%s failed: %s" % (what, RESULT_NAMES.get(code, code)))

@@ -1,0 +1,431 @@
+// Host runtime of the MI355X Weighted MinHash engine: the generator object,
+// device selection, parameter I/O, the grow-only device buffer cache and the
+// C ABI of include/minhashcuda.h.
+//
+// Reference counterparts: minhashcuda.cc (generator, init/calc/fini,
+// buffers), private.h (error and multi-device macros) and wrappers.h (RAII
+// device pointers).  Differences by design:
+//   * one GPU (the first gfx950 device the bitmask selects);
+//   * parameters live in the [dim][3][sp] column-block layout (see
+//     wmh_kernels.h); assign/retrieve re-lay them out on the device;
+//   * no host bin-packing: the kernels schedule rows themselves;
+//   * every call on one generator is serialized by a mutex (the reference
+//     mutates the generator through const pointers without locking);
+//   * `dim * samples` is computed in 64 bits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cinttypes>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "minhashcuda.h"
+#include "wmh_kernels.h"
+#include "wmh_runtime.h"
+
+#define INFO(...) do { if (verbosity > 0) { printf(__VA_ARGS__); fflush(stdout); } } while (false)
+#define DEBUG(...) do { if (verbosity > 1) { printf(__VA_ARGS__); fflush(stdout); } } while (false)
+
+// Evaluates a HIP call; on failure logs it and returns `ret`.
+#define HIPCH(call, ret)                                                      \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      INFO("%s:%d: %s -> %s\n", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return ret;                                                             \
+    }                                                                         \
+  } while (false)
+
+namespace wmh {
+
+DevBuf::~DevBuf() {
+  if (ptr) (void)hipFree(ptr);
+}
+
+hipError_t DevBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return hipSuccess;
+  if (ptr) {
+    (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+  hipError_t e = hipMalloc(&ptr, bytes);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return e;
+  }
+  cap = bytes;
+  return hipSuccess;
+}
+
+}  // namespace wmh
+
+using wmh::DevBuf;
+
+struct MinhashCudaGenerator_ {
+  uint32_t dim = 0;
+  uint16_t samples = 0;
+  int verbosity = 0;
+  int device = -1;
+  uint32_t sp = 0;
+  int num_cus = 0;
+  float *params = nullptr;            // [dim][3][sp]
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  // grow-only caches, reference minhashcuda.cc:27-31
+  DevBuf weights, cols, rows, hashes, staging, plan, scratch;
+  wmh::Tuning tuning{2, 2048, 8};
+  int kernel = mhcudaKernelAuto;
+
+  ~MinhashCudaGenerator_() {
+    if (device >= 0) (void)hipSetDevice(device);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (params) (void)hipFree(params);
+  }
+};
+
+namespace {
+
+// Selects the device: the bitmask semantics of minhashcuda.cc:43-79 (0 = all
+// devices, devices that fail validation are dropped), with the
+// compute-capability check replaced by a gfx950 architecture check.  The
+// engine drives one GPU: the first valid device is returned, -1 if none.
+int select_device(uint32_t devices, int verbosity) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  if (devices == 0) devices = count >= 32 ? 0xffffffffu : ((1u << count) - 1u);
+  int chosen = -1;
+  for (int dev = 0; devices && dev < 32; dev++, devices >>= 1) {
+    if (!(devices & 1u)) continue;
+    if (dev >= count) {
+      INFO("device %d does not exist\n", dev);
+      continue;
+    }
+    hipDeviceProp_t props;
+    if (hipGetDeviceProperties(&props, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      INFO("failed to query device %d\n", dev);
+      continue;
+    }
+    if (std::strncmp(props.gcnArchName, "gfx950", 6) != 0) {
+      INFO("device %d is %s, not gfx950: skipped\n", dev, props.gcnArchName);
+      continue;
+    }
+    if (chosen < 0) {
+      chosen = dev;
+    } else {
+      INFO("device %d also selected: this build hashes on one GPU (#%d)\n", dev, chosen);
+    }
+  }
+  return chosen;
+}
+
+MHCUDAResult print_memory_stats(int dev) {
+  size_t free_bytes = 0, total_bytes = 0;
+  if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) return mhcudaRuntimeError;
+  printf("GPU #%d memory: used %zu bytes (%.1f%%), free %zu bytes, total %zu bytes\n",
+         dev, total_bytes - free_bytes, (total_bytes - free_bytes) * 100.0 / total_bytes,
+         free_bytes, total_bytes);
+  return mhcudaSuccess;
+}
+
+// Columns per re-layout chunk: the staging table is samples x chunk floats,
+// at most 256 MiB.
+uint32_t relayout_chunk(uint32_t dim, uint32_t samples) {
+  uint64_t c = (256ull << 20) / (4ull * samples);
+  c = std::max<uint64_t>(64, c & ~63ull);
+  return (uint32_t)std::min<uint64_t>(c, dim);
+}
+
+}  // namespace
+
+namespace wmh {
+
+// Launches the chosen hashing kernel over device-resident matrices on `s`.
+MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen,
+                               std::vector<MatrixDesc> &mats, int kernel, hipStream_t s) {
+  int verbosity = gen->verbosity;
+  if (gen->samples % 2 != 0) {
+    // the reference finds no even sample_delta for odd counts
+    // (minhashcuda.cc:602-618) and rejects the call
+    return mhcudaInvalidArguments;
+  }
+  if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
+  if (kernel == mhcudaKernelAuto || kernel == mhcudaKernelPersistent) {
+    kernel = mhcudaKernelPlain;  // the persistent path is added separately
+  }
+  for (const MatrixDesc &m : mats) {
+    if (m.length == 0) continue;
+    HIPCH(launch_plain(gen->params, gen->dim, gen->samples, m, gen->tuning.spl, s),
+          mhcudaRuntimeError);
+  }
+  return mhcudaSuccess;
+}
+
+}  // namespace wmh
+
+extern "C" {
+
+MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
+                                  int deferred, uint32_t devices, int verbosity,
+                                  MHCUDAResult *status) {
+  DEBUG("mhcuda_init: %" PRIu32 " %" PRIu16 " %" PRIu32 " %d %" PRIu32 " %d %p\n",
+        dim, samples, seed, deferred, devices, verbosity, (void *)status);
+  if (dim == 0 || samples == 0) {
+    if (status) *status = mhcudaInvalidArguments;
+    return nullptr;
+  }
+  const int dev = select_device(devices, verbosity);
+  if (dev < 0) {
+    if (status) *status = mhcudaNoSuchDevice;
+    return nullptr;
+  }
+#define FAIL(code)                   \
+  do {                               \
+    if (status) *status = (code);    \
+    return nullptr;                  \
+  } while (false)
+  if (hipSetDevice(dev) != hipSuccess) FAIL(mhcudaNoSuchDevice);
+  std::unique_ptr<MinhashCudaGenerator> gen(new MinhashCudaGenerator());
+  gen->dim = dim;
+  gen->samples = samples;
+  gen->verbosity = verbosity;
+  gen->device = dev;
+  gen->sp = wmh::padded_samples(samples);
+  gen->tuning.spl = samples <= 64 ? 1 : (samples <= 128 ? 2 : 4);
+  hipDeviceProp_t props;
+  if (hipGetDeviceProperties(&props, dev) != hipSuccess) FAIL(mhcudaRuntimeError);
+  gen->num_cus = props.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&gen->stream, hipStreamNonBlocking) != hipSuccess)
+    FAIL(mhcudaRuntimeError);
+  const size_t param_bytes = (size_t)dim * 3u * gen->sp * sizeof(float);
+  if (hipMalloc(&gen->params, param_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    gen->params = nullptr;
+    INFO("failed to allocate %zu bytes for the parameters\n", param_bytes);
+    FAIL(mhcudaMemoryAllocationFailure);
+  }
+  if (deferred) {
+    if (hipMemsetAsync(gen->params, 0, param_bytes, gen->stream) != hipSuccess)
+      FAIL(mhcudaRuntimeError);
+  } else {
+    if (wmh::launch_gen_params(gen->params, dim, samples, seed, gen->stream) != hipSuccess)
+      FAIL(mhcudaRuntimeError);
+  }
+  if (hipStreamSynchronize(gen->stream) != hipSuccess) FAIL(mhcudaRuntimeError);
+  if (verbosity > 1 && print_memory_stats(dev) != mhcudaSuccess) FAIL(mhcudaRuntimeError);
+#undef FAIL
+  return gen.release();
+}
+
+MinhashCudaGeneratorParameters mhcuda_get_parameters(const MinhashCudaGenerator *gen) {
+  if (gen == nullptr) return MinhashCudaGeneratorParameters{};
+  MinhashCudaGeneratorParameters p{};
+  p.dim = gen->dim;
+  p.samples = gen->samples;
+  p.verbosity = gen->verbosity;
+  return p;
+}
+
+MHCUDAResult mhcuda_retrieve_random_vars(const MinhashCudaGenerator *cgen, float *rs,
+                                         float *ln_cs, float *betas) {
+  if (!cgen || !rs || !ln_cs || !betas) return mhcudaInvalidArguments;
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  std::lock_guard<std::mutex> lock(gen->mu);
+  int verbosity = gen->verbosity;
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  const uint32_t chunk = relayout_chunk(gen->dim, gen->samples);
+  HIPCH(gen->staging.reserve((size_t)chunk * gen->samples * sizeof(float)),
+        mhcudaMemoryAllocationFailure);
+  float *dst[3] = {rs, betas, ln_cs};  // layout order: 0 = rs, 1 = betas, 2 = ln_cs
+  for (uint32_t which = 0; which < 3; which++) {
+    for (uint32_t j0 = 0; j0 < gen->dim; j0 += chunk) {
+      const uint32_t n = std::min(chunk, gen->dim - j0);
+      HIPCH(wmh::launch_gather_params(gen->params, j0, n, which,
+                                      (float *)gen->staging.ptr, gen->samples, gen->stream),
+            mhcudaRuntimeError);
+      HIPCH(hipMemcpy2DAsync(dst[which] + j0, (size_t)gen->dim * sizeof(float),
+                             gen->staging.ptr, (size_t)n * sizeof(float),
+                             (size_t)n * sizeof(float), gen->samples,
+                             hipMemcpyDeviceToHost, gen->stream),
+            mhcudaMemoryCopyError);
+      HIPCH(hipStreamSynchronize(gen->stream), mhcudaMemoryCopyError);
+    }
+  }
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *cgen, const float *rs,
+                                       const float *ln_cs, const float *betas) {
+  if (!cgen || !rs || !ln_cs || !betas) return mhcudaInvalidArguments;
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  std::lock_guard<std::mutex> lock(gen->mu);
+  int verbosity = gen->verbosity;
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  const uint32_t chunk = relayout_chunk(gen->dim, gen->samples);
+  HIPCH(gen->staging.reserve((size_t)chunk * gen->samples * sizeof(float)),
+        mhcudaMemoryAllocationFailure);
+  const float *src[3] = {rs, betas, ln_cs};
+  for (uint32_t which = 0; which < 3; which++) {
+    for (uint32_t j0 = 0; j0 < gen->dim; j0 += chunk) {
+      const uint32_t n = std::min(chunk, gen->dim - j0);
+      HIPCH(hipMemcpy2DAsync(gen->staging.ptr, (size_t)n * sizeof(float),
+                             src[which] + j0, (size_t)gen->dim * sizeof(float),
+                             (size_t)n * sizeof(float), gen->samples,
+                             hipMemcpyHostToDevice, gen->stream),
+            mhcudaMemoryCopyError);
+      HIPCH(wmh::launch_scatter_params((const float *)gen->staging.ptr, n, j0, which,
+                                       gen->params, gen->samples, gen->stream),
+            mhcudaRuntimeError);
+      HIPCH(hipStreamSynchronize(gen->stream), mhcudaMemoryCopyError);
+    }
+  }
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_calc(const MinhashCudaGenerator *cgen, const float *weights,
+                         const uint32_t *cols, const uint32_t *rows, uint32_t length,
+                         uint32_t *output) {
+  if (!cgen || !weights || !cols || !rows || !output || length == 0)
+    return mhcudaInvalidArguments;
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  MHCUDAMatrix m{weights, cols, rows, length, output};
+  return mhcuda_calc_batch(gen, 1, &m, mhcudaKernelAuto);
+}
+
+MHCUDAResult mhcuda_fini(MinhashCudaGenerator *gen) {
+  delete gen;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_calc_device(const MinhashCudaGenerator *cgen, const float *weights,
+                                const uint32_t *cols, const uint32_t *rows,
+                                uint32_t length, uint32_t *output, void *stream) {
+  if (!cgen || !weights || !cols || !rows || !output || length == 0)
+    return mhcudaInvalidArguments;
+  MHCUDAMatrix m{weights, cols, rows, length, output};
+  return mhcuda_calc_batch_device(cgen, 1, &m, mhcudaKernelAuto, stream);
+}
+
+MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t count,
+                                      const MHCUDAMatrix *mats, int kernel, void *stream) {
+  if (!cgen || !mats || count == 0) return mhcudaInvalidArguments;
+  for (uint32_t m = 0; m < count; m++) {
+    if (mats[m].length && (!mats[m].weights || !mats[m].cols || !mats[m].rows || !mats[m].output))
+      return mhcudaInvalidArguments;
+  }
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  std::lock_guard<std::mutex> lock(gen->mu);
+  int verbosity = gen->verbosity;
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  std::vector<wmh::MatrixDesc> descs(count);
+  for (uint32_t m = 0; m < count; m++) {
+    descs[m] = wmh::MatrixDesc{mats[m].weights, mats[m].cols, mats[m].rows, mats[m].output,
+                               mats[m].length, 0u, 0u, 0u};
+  }
+  MHCUDAResult r = wmh::hash_device_batch(gen, descs, kernel, s);
+  if (r != mhcudaSuccess) return r;
+  if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
+                               const MHCUDAMatrix *mats, int kernel) {
+  if (!cgen || !mats || count == 0) return mhcudaInvalidArguments;
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  int verbosity = gen->verbosity;
+  uint64_t total_nnz = 0, total_rows = 0, total_out = 0;
+  for (uint32_t m = 0; m < count; m++) {
+    const MHCUDAMatrix &x = mats[m];
+    if (x.length == 0) continue;
+    if (!x.weights || !x.cols || !x.rows || !x.output) return mhcudaInvalidArguments;
+    if (x.rows[x.length] < x.rows[0]) return mhcudaInvalidArguments;
+    total_nnz += x.rows[x.length] - x.rows[0];
+    total_rows += (uint64_t)x.length + 1;
+    total_out += (uint64_t)x.length * gen->samples * 2;
+  }
+  if (total_rows == 0) return mhcudaInvalidArguments;
+  if (gen->samples % 2 != 0) return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  DEBUG("mhcuda_calc: %u matrices, %" PRIu64 " rows, %" PRIu64 " nonzeros\n", count,
+        total_rows - count, total_nnz);
+  // reference fill_buffers (minhashcuda.cc:385-455): grow-only device buffers
+  HIPCH(gen->weights.reserve(std::max<uint64_t>(total_nnz, 1) * 4), mhcudaMemoryAllocationFailure);
+  HIPCH(gen->cols.reserve(std::max<uint64_t>(total_nnz, 1) * 4), mhcudaMemoryAllocationFailure);
+  HIPCH(gen->rows.reserve(total_rows * 4), mhcudaMemoryAllocationFailure);
+  HIPCH(gen->hashes.reserve(std::max<uint64_t>(total_out, 1) * 4), mhcudaMemoryAllocationFailure);
+  std::vector<wmh::MatrixDesc> dev(count);
+  uint64_t nnz_off = 0, row_off = 0, out_off = 0;
+  float *dw = (float *)gen->weights.ptr;
+  uint32_t *dc = (uint32_t *)gen->cols.ptr, *dr = (uint32_t *)gen->rows.ptr,
+           *dh = (uint32_t *)gen->hashes.ptr;
+  for (uint32_t m = 0; m < count; m++) {
+    const MHCUDAMatrix &x = mats[m];
+    dev[m] = wmh::MatrixDesc{nullptr, nullptr, nullptr, nullptr, 0u, 0u, 0u, 0u};
+    if (x.length == 0) continue;
+    const uint64_t nnz = x.rows[x.length] - x.rows[0];
+    // the device copy of the matrix starts at CSR position rows[0]
+    if (nnz) {
+      HIPCH(hipMemcpyAsync(dw + nnz_off, x.weights + x.rows[0], nnz * 4,
+                           hipMemcpyHostToDevice, gen->stream), mhcudaMemoryCopyError);
+      HIPCH(hipMemcpyAsync(dc + nnz_off, x.cols + x.rows[0], nnz * 4,
+                           hipMemcpyHostToDevice, gen->stream), mhcudaMemoryCopyError);
+    }
+    HIPCH(hipMemcpyAsync(dr + row_off, x.rows, ((uint64_t)x.length + 1) * 4,
+                         hipMemcpyHostToDevice, gen->stream), mhcudaMemoryCopyError);
+    dev[m].weights = dw + nnz_off;
+    dev[m].cols = dc + nnz_off;
+    dev[m].rows = dr + row_off;
+    dev[m].length = x.length;
+    dev[m].base = x.rows[0];
+    dev[m].output = dh + out_off;
+    nnz_off += nnz;
+    row_off += (uint64_t)x.length + 1;
+    out_off += (uint64_t)x.length * gen->samples * 2;
+  }
+  MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream);
+  if (r != mhcudaSuccess) return r;
+  for (uint32_t m = 0; m < count; m++) {
+    if (mats[m].length == 0) continue;
+    HIPCH(hipMemcpyAsync(mats[m].output, dev[m].output,
+                         (uint64_t)mats[m].length * gen->samples * 2 * 4,
+                         hipMemcpyDeviceToHost, gen->stream), mhcudaMemoryCopyError);
+  }
+  HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
+  INFO("mhcuda - success\n");
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_set_tuning(MinhashCudaGenerator *gen, int samples_per_lane,
+                               int segment, int waves_per_cu) {
+  if (!gen) return mhcudaInvalidArguments;
+  if (samples_per_lane && samples_per_lane != 1 && samples_per_lane != 2 && samples_per_lane != 4)
+    return mhcudaInvalidArguments;
+  if (segment < 0 || (segment % 64) != 0 || waves_per_cu < 0 || waves_per_cu > 32)
+    return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  if (samples_per_lane) gen->tuning.spl = samples_per_lane;
+  if (segment) gen->tuning.seg = segment;
+  if (waves_per_cu) gen->tuning.waves_per_cu = waves_per_cu;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen, int *samples_per_lane,
+                               int *segment, int *waves_per_cu) {
+  if (!gen) return mhcudaInvalidArguments;
+  if (samples_per_lane) *samples_per_lane = gen->tuning.spl;
+  if (segment) *segment = gen->tuning.seg;
+  if (waves_per_cu) *waves_per_cu = gen->tuning.waves_per_cu;
+  return mhcudaSuccess;
+}
+
+}  // extern "C"

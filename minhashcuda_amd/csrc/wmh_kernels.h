@@ -1,0 +1,55 @@
+// Internal launcher interface between the host runtime (generator.cpp) and the
+// gfx950 kernels (wmh_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wmh {
+
+// Parameter table layout in HBM: one contiguous block per column j,
+//   params[j * 3 * sp + which * sp + k],  which: 0 = rs, 1 = betas, 2 = ln_cs,
+// with sp = samples rounded up to a multiple of 4 (pad entries are zero).
+// A nonzero's parameters for every sample are therefore one 3*sp*4-byte run
+// (1.5 KiB at 128 samples) read with fully coalesced wavefront loads.
+inline uint32_t padded_samples(uint32_t samples) { return (samples + 3u) & ~3u; }
+
+// One CSR matrix of a batch (all device pointers).  rows has length + 1
+// entries; CSR position p lives at weights[p - base] / cols[p - base].
+struct MatrixDesc {
+  const float *weights;
+  const uint32_t *cols;
+  const uint32_t *rows;
+  uint32_t *output;       // [length][samples][2]
+  uint32_t length;
+  uint32_t base;
+  uint32_t item_begin;    // first work item of this matrix (persistent kernel)
+  uint32_t n_items;       // work items covering this matrix
+};
+
+// Kernel tuning knobs (chosen by the autotuner).
+struct Tuning {
+  int spl;          // samples per lane: 1, 2 or 4
+  int seg;          // nonzeros per persistent work item (multiple of 64)
+  int waves_per_cu; // persistent grid size = CUs * waves_per_cu
+};
+
+hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
+                             uint32_t seed, hipStream_t stream);
+
+// Scatter one [samples][ncols] FP32 table (rows of pitch ncols) into
+// params[(j0 + jj) * 3 * sp + which * sp + k].
+hipError_t launch_scatter_params(const float *staging, uint32_t ncols,
+                                 uint32_t j0, uint32_t which, float *params,
+                                 uint32_t samples, hipStream_t stream);
+// The inverse: gather params back into a [samples][ncols] staging table.
+hipError_t launch_gather_params(const float *params, uint32_t j0, uint32_t ncols,
+                                uint32_t which, float *staging, uint32_t samples,
+                                hipStream_t stream);
+
+// One launch hashing every row of one matrix: one wavefront per
+// (row, sample tile).
+hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
+                        const MatrixDesc &m, int spl, hipStream_t stream);
+
+}  // namespace wmh

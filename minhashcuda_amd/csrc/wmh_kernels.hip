@@ -1,0 +1,393 @@
+// gfx950 (MI355X, CDNA4) kernels of the Weighted MinHash engine.
+//
+// Hot path (reference: kernel.cu:56-112, weighted_minhash_cuda).  For a CSR
+// row with nonzeros (j_p, w_p) in CSR order and every sample k:
+//   lw   = ln(w_p)                       (once per nonzero, wmh_logf)
+//   t    = floor(lw / r[k,j] + beta[k,j])
+//   ln_a = (ln_c[k,j] - (t - beta[k,j]) * r[k,j]) - r[k,j]
+//   keep the first p with the smallest ln_a (strict <, initial FLT_MAX),
+//   output (j_p, (uint32)(int32)t).
+// All arithmetic is binary32, unfused (this file is compiled with
+// -ffp-contract=off), with IEEE division and preserved denormals.
+//
+// Mapping to CDNA4: one wavefront owns one (row, sample tile).  Lane l holds
+// V consecutive samples, so a wavefront covers 64*V samples with running
+// argmins in VGPRs and never needs a cross-lane reduction.  The row's
+// (col, w) pairs are read 64 at a time with one coalesced load per lane, each
+// lane takes the logarithm of its own weight, and the pairs are then
+// broadcast to SGPRs with v_readlane: the column becomes a scalar base address
+// and each nonzero costs three coalesced 64*V*4-byte loads from the column's
+// contiguous parameter block.  No MFMA: this is a gather + compare, not a
+// contraction; the kernel is bound by HBM bandwidth on those gathers.
+#include <float.h>
+
+#include "wmh_kernels.h"
+#include "wmh_math.h"
+
+namespace wmh {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;          // 4 wavefronts per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t i) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)i);
+}
+
+__device__ __forceinline__ float readlane_f32(float v, uint32_t i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)i));
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// ---------------------------------------------------------------------------
+// Parameter generation: Philox4x32-10 (Salmon et al., SC'11) + Gamma(2,1).
+// Replaces the reference's cuRAND + gamma_cuda/log_cuda (kernel.cu:18-36,
+// minhashcuda.cc:176-200) with a counter-based generator whose stream is a
+// pure function of (seed, k, j), so any device -- or the CPU -- regenerates
+// the same table without communication.
+
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int round = 0; round < 10; round++) {
+    if (round > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+constexpr uint32_t kPhiloxKey1 = 0x4d484344u;  // "MHCD"
+
+__device__ __forceinline__ float u_open(uint32_t x) {
+  return (float)((x >> 9) * 2u + 1u) * 0x1p-24f;  // odd multiples of 2^-24 in (0,1)
+}
+__device__ __forceinline__ float u_half_open(uint32_t x) {
+  return (float)(x >> 8) * 0x1p-24f;  // multiples of 2^-24 in [0,1)
+}
+
+__global__ __launch_bounds__(kBlock) void gen_params_kernel(
+    float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
+    uint32_t seed) {
+  WMH_NO_CONTRACT
+  const uint64_t total = (uint64_t)dim * sp;
+  for (uint64_t idx = (uint64_t)blockIdx.x * kBlock + threadIdx.x; idx < total;
+       idx += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t j = (uint32_t)(idx / sp);
+    const uint32_t k = (uint32_t)(idx - (uint64_t)j * sp);
+    float r = 0.0f, beta = 0.0f, lc = 0.0f;
+    if (k < samples) {
+      const uint64_t n = (uint64_t)k * dim + j;  // logical [samples][dim] index
+      const U4 a = philox4x32_10(U4{(uint32_t)n, (uint32_t)(n >> 32), 0u, 0u}, seed, kPhiloxKey1);
+      const U4 b = philox4x32_10(U4{(uint32_t)n, (uint32_t)(n >> 32), 1u, 0u}, seed, kPhiloxKey1);
+      r = -wmh_logf(u_open(a.x) * u_open(a.y));
+      lc = wmh_logf(-wmh_logf(u_open(a.z) * u_open(a.w)));
+      beta = u_half_open(b.x);
+    }
+    float *col = params + (uint64_t)j * 3u * sp;
+    col[k] = r;
+    col[sp + k] = beta;
+    col[2u * sp + k] = lc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Re-layout between the reference's [samples][dim] tables (kernel.cu:97-99,
+// python.cc:157) and the internal [dim][3][sp] blocks: 64x64 LDS tiles so that
+// both the read and the write side are coalesced.
+
+__global__ __launch_bounds__(kBlock) void scatter_params_kernel(
+    const float *__restrict__ staging, uint32_t ncols, uint32_t j0,
+    uint32_t which, float *__restrict__ params, uint32_t samples, uint32_t sp) {
+  __shared__ float tile[64][65];
+  const uint32_t jj0 = blockIdx.x * 64u, k0 = blockIdx.y * 64u;
+  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  for (uint32_t r = ty; r < 64u; r += kWavesPerBlock) {
+    const uint32_t k = k0 + r, jj = jj0 + tx;
+    tile[r][tx] = (k < samples && jj < ncols) ? staging[(uint64_t)k * ncols + jj] : 0.0f;
+  }
+  __syncthreads();
+  for (uint32_t r = ty; r < 64u; r += kWavesPerBlock) {
+    const uint32_t jj = jj0 + r, k = k0 + tx;
+    if (jj < ncols && k < sp)
+      params[(uint64_t)(j0 + jj) * 3u * sp + which * sp + k] = tile[tx][r];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void gather_params_kernel(
+    const float *__restrict__ params, uint32_t j0, uint32_t ncols,
+    uint32_t which, float *__restrict__ staging, uint32_t samples, uint32_t sp) {
+  __shared__ float tile[64][65];
+  const uint32_t jj0 = blockIdx.x * 64u, k0 = blockIdx.y * 64u;
+  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  for (uint32_t r = ty; r < 64u; r += kWavesPerBlock) {
+    const uint32_t jj = jj0 + r, k = k0 + tx;
+    tile[r][tx] = (jj < ncols && k < samples)
+                      ? params[(uint64_t)(j0 + jj) * 3u * sp + which * sp + k]
+                      : 0.0f;
+  }
+  __syncthreads();
+  for (uint32_t r = ty; r < 64u; r += kWavesPerBlock) {
+    const uint32_t k = k0 + r, jj = jj0 + tx;
+    if (k < samples && jj < ncols) staging[(uint64_t)k * ncols + jj] = tile[tx][r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The hashing core.
+
+template <int V>
+struct Argmin {
+  float best[V];   // smallest ln_a so far (FLT_MAX: nothing selected)
+  float t[V];      // t of the selected nonzero
+  uint32_t j[V];   // its column (WMH_SENTINEL: nothing selected)
+
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      best[v] = FLT_MAX;
+      t[v] = 0.0f;
+      j[v] = WMH_SENTINEL;
+    }
+  }
+};
+
+// Loads V consecutive floats starting at p (8/16-byte aligned for V = 2/4).
+template <int V>
+__device__ __forceinline__ void load_v(const float *__restrict__ p, float (&out)[V]) {
+  if constexpr (V == 1) {
+    out[0] = p[0];
+  } else if constexpr (V == 2) {
+    const float2 x = *reinterpret_cast<const float2 *>(p);
+    out[0] = x.x;
+    out[1] = x.y;
+  } else {
+    const float4 x = *reinterpret_cast<const float4 *>(p);
+    out[0] = x.x;
+    out[1] = x.y;
+    out[2] = x.z;
+    out[3] = x.w;
+  }
+}
+
+// Folds one nonzero (column j, log-weight lw, parameters r/beta/lc for the
+// lane's V samples) into the running argmins.  `ok` (wave-uniform) is false
+// for columns >= dim, which are never selected.
+template <int V>
+__device__ __forceinline__ void fold(const float (&r)[V], const float (&beta)[V],
+                                     const float (&lc)[V], uint32_t j, float lw,
+                                     bool ok, Argmin<V> &acc) {
+  WMH_NO_CONTRACT
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    const float t = __builtin_floorf(lw / r[v] + beta[v]);   // kernel.cu:103
+    const float ln_y = (t - beta[v]) * r[v];                 // kernel.cu:104
+    const float ln_a = (lc[v] - ln_y) - r[v];                // kernel.cu:105
+    const bool upd = ok && ln_a < acc.best[v];               // kernel.cu:106
+    acc.best[v] = upd ? ln_a : acc.best[v];
+    acc.t[v] = upd ? t : acc.t[v];
+    acc.j[v] = upd ? j : acc.j[v];
+  }
+}
+
+// Byte offsets of the lane's r / beta / ln_c values inside a column block.
+struct LaneOffsets {
+  uint32_t r, beta, lc;
+};
+
+__device__ __forceinline__ const float *at(const char *base, uint32_t off) {
+  return reinterpret_cast<const float *>(base + off);
+}
+
+// Hashes CSR positions [begin, end) of one row into `acc` (CSR order, so a
+// tie keeps the earliest position).  Nonzeros are folded in groups of U whose
+// 3*U gathers are all issued before the first one is consumed.
+template <int V, int U>
+__device__ __forceinline__ void hash_range(const float *__restrict__ params,
+                                           uint32_t dim, uint64_t block_bytes,
+                                           LaneOffsets lo,
+                                           const float *__restrict__ weights,
+                                           const uint32_t *__restrict__ cols,
+                                           uint32_t begin, uint32_t end,
+                                           Argmin<V> &acc) {
+  const uint32_t lane = lane_id();
+  const char *pbase = reinterpret_cast<const char *>(params);
+  for (uint32_t base = begin; base < end; base += kWave) {
+    const uint32_t n = uniform(min((uint32_t)kWave, end - base));
+    uint32_t c = WMH_SENTINEL;
+    float lw = 0.0f;
+    if (lane < n) {
+      c = cols[base + lane];
+      lw = wmh_logf(weights[base + lane]);  // once per nonzero (kernel.cu:95)
+    }
+    uint32_t i = 0;
+    for (; i + U <= n; i += U) {
+      uint32_t j[U];
+      float lwu[U];
+      float r[U][V], beta[U][V], lc[U][V];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        j[u] = readlane_u32(c, i + u);
+        lwu[u] = readlane_f32(lw, i + u);
+        const char *blk = pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes;
+        load_v<V>(at(blk, lo.r), r[u]);
+        load_v<V>(at(blk, lo.beta), beta[u]);
+        load_v<V>(at(blk, lo.lc), lc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) fold<V>(r[u], beta[u], lc[u], j[u], lwu[u], j[u] < dim, acc);
+    }
+    for (; i < n; i++) {
+      const uint32_t j = readlane_u32(c, i);
+      const float lwi = readlane_f32(lw, i);
+      const char *blk = pbase + (uint64_t)(j < dim ? j : 0u) * block_bytes;
+      float r[V], beta[V], lc[V];
+      load_v<V>(at(blk, lo.r), r);
+      load_v<V>(at(blk, lo.beta), beta);
+      load_v<V>(at(blk, lo.lc), lc);
+      fold<V>(r, beta, lc, j, lwi, j < dim, acc);
+    }
+  }
+}
+
+// Writes the lane's V hashes of `row` (kernel.cu:83-85: out[(row*S + k)*2] =
+// column, out[... + 1] = (uint32)(int32)t).
+template <int V>
+__device__ __forceinline__ void write_hashes(uint32_t *__restrict__ output,
+                                             uint64_t row, uint32_t samples,
+                                             uint32_t k0, const Argmin<V> &acc) {
+  uint32_t h[2 * V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    const bool sel = acc.j[v] != WMH_SENTINEL;
+    h[2 * v] = acc.j[v];
+    h[2 * v + 1] = sel ? wmh_t_bits(acc.t[v]) : WMH_SENTINEL;
+  }
+  uint32_t *dst = output + (row * samples + k0) * 2u;
+  if (k0 + V <= samples) {
+    if constexpr (V == 1) {
+      *reinterpret_cast<uint2 *>(dst) = make_uint2(h[0], h[1]);
+    } else if constexpr (V == 2) {
+      *reinterpret_cast<uint4 *>(dst) = make_uint4(h[0], h[1], h[2], h[3]);
+    } else {
+      reinterpret_cast<uint4 *>(dst)[0] = make_uint4(h[0], h[1], h[2], h[3]);
+      reinterpret_cast<uint4 *>(dst)[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      if (k0 + v < samples) {
+        dst[2 * v] = h[2 * v];
+        dst[2 * v + 1] = h[2 * v + 1];
+      }
+    }
+  }
+}
+
+// Plain kernel: wavefront w of the grid handles (row, tile) = divmod(w, ntiles)
+// for the whole row.  Reference: kernel.cu:56-112.
+template <int V>
+__global__ __launch_bounds__(kBlock) void wmh_plain_kernel(
+    const float *__restrict__ params, uint32_t dim, uint32_t samples,
+    uint32_t sp, uint32_t ntiles, const float *__restrict__ weights,
+    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ rows,
+    uint32_t length, uint32_t base0, uint32_t *__restrict__ output) {
+  const uint64_t total = (uint64_t)length * ntiles;
+  const uint32_t lane = lane_id();
+  for (uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+       w < total; w += (uint64_t)gridDim.x * kWavesPerBlock) {
+    const uint32_t row = uniform((uint32_t)(w / ntiles));
+    const uint32_t tile = uniform((uint32_t)(w - (uint64_t)row * ntiles));
+    const uint32_t begin = uniform(rows[row] - base0);
+    const uint32_t end = uniform(rows[row + 1] - base0);
+    const uint32_t k0 = tile * (kWave * V) + lane * V;
+    const uint32_t koff = (k0 < samples) ? k0 : 0u;
+    const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
+    Argmin<V> acc;
+    acc.init();
+    hash_range<V, 16 / V>(params, dim, 12ull * sp, lo, weights, cols, begin, end, acc);
+    if (k0 < samples) write_hashes<V>(output, row, samples, k0, acc);
+  }
+}
+
+unsigned grid_for(uint64_t waves) {
+  const uint64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  return (unsigned)(blocks < (1u << 20) ? (blocks ? blocks : 1) : (1u << 20));
+}
+
+}  // namespace
+
+hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
+                             uint32_t seed, hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  const uint64_t total = (uint64_t)dim * sp;
+  uint64_t blocks = (total + kBlock - 1) / kBlock;
+  if (blocks > 65536u * 4u) blocks = 65536u * 4u;
+  hipLaunchKernelGGL(gen_params_kernel, dim3((unsigned)blocks), dim3(kBlock), 0,
+                     stream, params, dim, samples, sp, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_params(const float *staging, uint32_t ncols,
+                                 uint32_t j0, uint32_t which, float *params,
+                                 uint32_t samples, hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  dim3 grid((ncols + 63) / 64, (sp + 63) / 64);
+  hipLaunchKernelGGL(scatter_params_kernel, grid, dim3(kBlock), 0, stream,
+                     staging, ncols, j0, which, params, samples, sp);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_params(const float *params, uint32_t j0, uint32_t ncols,
+                                uint32_t which, float *staging, uint32_t samples,
+                                hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  dim3 grid((ncols + 63) / 64, (samples + 63) / 64);
+  hipLaunchKernelGGL(gather_params_kernel, grid, dim3(kBlock), 0, stream,
+                     params, j0, ncols, which, staging, samples, sp);
+  return hipGetLastError();
+}
+
+hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
+                        const MatrixDesc &m, int spl, hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  const uint32_t per_wave = (uint32_t)(kWave * spl);
+  const uint32_t ntiles = (samples + per_wave - 1) / per_wave;
+  const unsigned grid = grid_for((uint64_t)m.length * ntiles);
+  switch (spl) {
+    case 1:
+      hipLaunchKernelGGL(wmh_plain_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
+                         params, dim, samples, sp, ntiles, m.weights, m.cols, m.rows,
+                         m.length, m.base, m.output);
+      break;
+    case 2:
+      hipLaunchKernelGGL(wmh_plain_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
+                         params, dim, samples, sp, ntiles, m.weights, m.cols, m.rows,
+                         m.length, m.base, m.output);
+      break;
+    case 4:
+      hipLaunchKernelGGL(wmh_plain_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
+                         params, dim, samples, sp, ntiles, m.weights, m.cols, m.rows,
+                         m.length, m.base, m.output);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace wmh

@@ -1,0 +1,255 @@
+"""CPU tests that pin the oracle (no GPU needed).
+
+Each check is chosen so that a plausible mistake in the oracle -- a dropped
+term, a wrong operation order, a transposed parameter table, the wrong
+tie-break -- fails at least one of them:
+
+* the logarithm: exhaustively against an 80-bit logl over every binary32
+  input, plus special values;
+* Philox4x32-10: the Random123 known-answer vectors;
+* the formula and the argmin: against an independent numpy float32
+  restatement of the reference tests' oracle (datasketch's minhash), against
+  closed forms (single-nonzero rows, w = 1), against FP64 brute force, and on
+  the statistical property the hash exists for (P[equal] = weighted Jaccard);
+* the committed golden vectors.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.stats import gamma, uniform
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+S = oracle.SENTINEL
+
+
+def load_golden(name):
+    return dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
+
+
+# ---------------------------------------------------------------- logarithm
+
+def test_logf_exhaustive_correctly_rounded():
+    """wmh_logf(x) == RN(ln x) for every non-negative binary32 x."""
+    res = subprocess.run([oracle.LOGF_CHECK], stdout=subprocess.PIPE, text=True, check=True)
+    r = json.loads(res.stdout)
+    assert r["n"] == 0x7F800001
+    assert r["misrounded"] == 0, r
+    assert r["specials_bad"] == 0
+    assert r["hard"] == 0  # the 80-bit reference itself rounds unambiguously
+    assert r["max_ulp"] <= 0.5 + 1e-6
+
+
+def test_logf_specials():
+    x = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -1.0, -1e-40, 1.0, 2.0, 1e-45],
+                 dtype=np.float32)
+    y = oracle.logf(x)
+    assert y[0] == -np.inf and y[1] == -np.inf
+    assert y[2] == np.inf
+    assert np.all(np.isnan(y[3:7]))
+    assert y[7] == 0.0 and not np.signbit(y[7])
+    assert y[8] == np.float32(np.log(2.0))
+    assert y[9] == np.float32(np.log(np.float64(np.float32(1e-45))))
+
+
+def test_logf_matches_float64_rounding_on_random_inputs():
+    rng = np.random.default_rng(5)
+    bits = rng.integers(1, 0x7F800000, size=200000, dtype=np.uint32)
+    x = bits.view(np.float32)
+    ref = np.log(x.astype(np.float64)).astype(np.float32)
+    got = oracle.logf(x)
+    # double rounding through float64 can differ only on near-ties; none here
+    assert np.array_equal(got, ref)
+
+
+# ---------------------------------------------------------------- Philox
+
+@pytest.mark.parametrize("ctr,key,expected", [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+])
+def test_philox_known_answers(ctr, key, expected):
+    assert list(oracle.philox(ctr, key)) == expected
+
+
+def test_generated_parameter_distributions():
+    """test.py:72-90: Gamma(2,1) fits of rs and exp(ln_cs), uniform betas."""
+    rs, ln_cs, betas = oracle.gen_params(1, 1000, 128)
+    assert rs.shape == ln_cs.shape == betas.shape == (128, 1000)
+    a, loc, scale = gamma.fit(rs.ravel())
+    assert 1.97 < a < 2.03 and -0.01 < loc < 0.01 and 0.98 < scale < 1.02
+    a, loc, scale = gamma.fit(np.exp(ln_cs.ravel()))
+    assert 1.97 < a < 2.03 and -0.01 < loc < 0.01 and 0.98 < scale < 1.02
+    bmin, bmax = uniform.fit(betas.ravel())
+    assert 0 <= bmin < 0.001 and 0.999 <= bmax <= 1
+    assert np.all(rs > 0) and np.all(np.isfinite(ln_cs))
+    assert np.all((betas >= 0) & (betas < 1))
+
+
+def test_generated_parameters_are_a_pure_function_of_seed():
+    a = oracle.gen_params(9, 50, 16)
+    b = oracle.gen_params(9, 50, 16)
+    c = oracle.gen_params(10, 50, 16)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert not np.array_equal(a[0], c[0])
+    # element (k, j) does not depend on the table shape (counter = k*dim + j)
+    small = oracle.gen_params(9, 50, 8)
+    assert np.array_equal(small[0], a[0][:8])
+
+
+# ---------------------------------------------------------------- formula
+
+def test_tiny_matches_numpy_restatement():
+    g = load_golden("tiny.npz")
+    m = sp.csr_matrix((g["data"], g["indices"], g["indptr"]), shape=tuple(g["shape"]))
+    h = oracle.calc_csr(g["rs"], g["ln_cs"], g["betas"], m)
+    assert h.shape == (2, 128, 2)
+    assert np.array_equal(h, oracle.numpy_minhash(m.toarray(), g["rs"], g["ln_cs"], g["betas"]))
+    assert np.array_equal(h, g["hashes"])
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_c1_matches_numpy_restatement(seed):
+    """The 6400x130 recipe (test.py:35-38, README.md:85-90), all rows."""
+    data = oracle.c1_data(seed)
+    m = sp.csr_matrix(data, dtype=np.float32)
+    assert seed != 1 or m.nnz == 164594
+    rs, ln_cs, betas = oracle.datasketch_style_params(130, 128, seed=1)
+    h = oracle.calc_csr(rs, ln_cs, betas, m)
+    assert np.array_equal(h, oracle.numpy_minhash(data, rs, ln_cs, betas))
+
+
+def test_float_vector_negative_t():
+    g = load_golden("float.npz")
+    m = sp.csr_matrix((g["data"], g["indices"], g["indptr"]), shape=tuple(g["shape"]))
+    h = oracle.calc_csr(g["rs"], g["ln_cs"], g["betas"], m)
+    assert np.array_equal(h, g["hashes"])
+    assert (h[0, :, 1].view(np.int32) < 0).any()  # weights < 1 give negative t
+    assert np.array_equal(h, oracle.numpy_minhash(m.toarray(), g["rs"], g["ln_cs"], g["betas"]))
+
+
+def test_closed_form_single_nonzero():
+    """A one-nonzero row selects its column with t = floor(ln w / r + beta)."""
+    rng = np.random.default_rng(1)
+    dim, samples = 40, 32
+    rs, ln_cs, betas = oracle.datasketch_style_params(dim, samples, seed=4)
+    cols = rng.integers(0, dim, 50).astype(np.uint32)
+    w = rng.uniform(0.01, 100, 50).astype(np.float32)
+    indptr = np.arange(51, dtype=np.uint32)
+    h = oracle.calc(rs, ln_cs, betas, w, cols, indptr)
+    lw = oracle.logf(w)
+    for i in range(50):
+        j = cols[i]
+        t = np.floor(lw[i] / rs[:, j] + betas[:, j]).astype(np.int32)
+        assert np.array_equal(h[i, :, 0], np.full(samples, j))
+        assert np.array_equal(h[i, :, 1].view(np.int32), t)
+
+
+def test_closed_form_unit_weights():
+    """w = 1 gives ln w = 0 exactly, so t = floor(beta) = 0 and the argmin is
+    over ln_c + beta*r - r -- independent of any logarithm implementation."""
+    dim, samples = 30, 64
+    rs, ln_cs, betas = oracle.datasketch_style_params(dim, samples, seed=5)
+    cols = np.array([2, 5, 11, 17, 29], np.uint32)
+    w = np.ones(5, np.float32)
+    h = oracle.calc(rs, ln_cs, betas, w, cols, np.array([0, 5], np.uint32))
+    assert (h[0, :, 1] == 0).all()
+    with np.errstate(all="ignore"):
+        ln_a = ln_cs[:, cols] - (np.float32(0) - betas[:, cols]) * rs[:, cols] - rs[:, cols]
+    assert np.array_equal(h[0, :, 0], cols[np.argmin(ln_a, axis=1)])
+
+
+def test_fp64_brute_force_agreement():
+    """FP32 rounding rarely flips the argmin relative to FP64: a wrong term or
+    sign in the FP32 path would disagree on most samples."""
+    data = oracle.c1_data(2, shape=(500, 60))
+    m = sp.csr_matrix(data, dtype=np.float32)
+    rs, ln_cs, betas = oracle.datasketch_style_params(60, 128, seed=2)
+    h32 = oracle.calc_csr(rs, ln_cs, betas, m)
+    h64 = oracle.calc_f64(rs, ln_cs, betas, m.data, m.indices, m.indptr)
+    flips = np.mean(np.any(h32 != h64, axis=2))
+    assert flips < 1e-3
+
+
+def test_edge_rows_golden():
+    g = load_golden("edges.npz")
+    h = oracle.calc(g["rs"], g["ln_cs"], g["betas"], g["data"], g["indices"], g["indptr"])
+    assert np.array_equal(h, g["hashes"])
+    assert (h[0] == S).all()          # empty row
+    assert (h[2] == S).all()          # explicit zeros only
+    assert (h[3] == S).all()          # negative / NaN only
+    assert (h[7, :, 0] == 5).all()    # duplicate column: earliest position wins
+
+
+def test_ties_keep_earliest_position_and_signed_zero():
+    """Two nonzeros with bit-identical parameters tie on every sample: the
+    earlier CSR position must win (strict <).  -0 and +0 weights' logs are
+    -inf (never selected)."""
+    dim, samples = 4, 16
+    rs, ln_cs, betas = oracle.datasketch_style_params(dim, samples, seed=6)
+    for p in (rs, ln_cs, betas):
+        p[:, 2] = p[:, 1]
+    w = np.array([3.0, 3.0], np.float32)
+    h = oracle.calc(rs, ln_cs, betas, w, np.array([2, 1], np.uint32), np.array([0, 2], np.uint32))
+    assert (h[0, :, 0] == 2).all()
+    h = oracle.calc(rs, ln_cs, betas, np.array([-0.0, 0.0], np.float32),
+                    np.array([0, 1], np.uint32), np.array([0, 2], np.uint32))
+    assert (h == S).all()
+
+
+def test_t_saturation():
+    """(int32)t saturates: a huge weight over a tiny r overflows int32."""
+    rs = np.full((8, 2), 1e-30, np.float32)
+    ln_cs = np.zeros((8, 2), np.float32)
+    betas = np.zeros((8, 2), np.float32)
+    h = oracle.calc(rs, ln_cs, betas, np.array([1e30], np.float32), np.array([1], np.uint32),
+                    np.array([0, 1], np.uint32))
+    assert (h[0, :, 1] == 0x7FFFFFFF).all()
+
+
+def test_row_independence_and_permutation():
+    """Each row's hash depends only on that row: slicing and permuting rows
+    permute the output (test.py:112-124)."""
+    data = oracle.c1_data(0, shape=(300, 50))
+    m = sp.csr_matrix(data, dtype=np.float32)
+    rs, ln_cs, betas = oracle.datasketch_style_params(50, 64, seed=8)
+    h = oracle.calc_csr(rs, ln_cs, betas, m)
+    perm = np.random.default_rng(0).permutation(300)
+    assert np.array_equal(oracle.calc_csr(rs, ln_cs, betas, m[perm]), h[perm])
+    assert np.array_equal(oracle.calc_csr(rs, ln_cs, betas, m[100:200]), h[100:200])
+
+
+def test_omp_equals_single_thread():
+    data = oracle.c1_data(1)
+    m = sp.csr_matrix(data, dtype=np.float32)
+    rs, ln_cs, betas = oracle.datasketch_style_params(130, 128, seed=1)
+    h = oracle.calc_csr(rs, ln_cs, betas, m)
+    h2, used = oracle.calc_omp(rs, ln_cs, betas, m.data, m.indices, m.indptr, 4)
+    assert used >= 1 and np.array_equal(h, h2)
+
+
+def test_jaccard_estimator():
+    """P[(k,t) equal] = weighted Jaccard (consistent weighted sampling); the
+    mean error over many pairs at 128 samples is small (test.py:92-110 checks
+    a single pair)."""
+    rng = np.random.default_rng(11)
+    dim, samples, pairs = 200, 128, 200
+    rs, ln_cs, betas = oracle.gen_params(1, dim, samples)
+    base = rng.uniform(0, 10, (pairs, dim)) * (rng.uniform(size=(pairs, dim)) < 0.3)
+    other = base * rng.uniform(0.5, 1.5, (pairs, dim))
+    other *= rng.uniform(size=(pairs, dim)) < 0.8
+    rows = np.vstack([base, other]).astype(np.float32)
+    m = sp.csr_matrix(rows)
+    h = oracle.calc_csr(rs, ln_cs, betas, m)
+    est = np.array([oracle.hash_jaccard(h[i], h[pairs + i]) for i in range(pairs)])
+    true = np.array([oracle.weighted_jaccard(rows[i], rows[pairs + i]) for i in range(pairs)])
+    err = est - true
+    assert abs(err.mean()) < 0.01          # unbiased
+    assert np.sqrt(np.mean(err ** 2)) < 0.06  # ~ sqrt(J(1-J)/128)
