@@ -1,17 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the Weighted MinHash hashing path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched|plain]
 
-A step is one hashing pass over one batch of synthetic CSR input that is
-already resident in HBM (device entry point of the C ABI, launched on torch's
-current stream).  Prints ONE JSON line with the metric nnz*samples/s, the
-roofline of the dominant kernel (HBM-bound parameter gathers) and the CPU
-oracle baseline timed on this host.
+A step is one hashing pass over one batch of synthetic CSR input already
+resident in HBM (device entry points of the C ABI, launched on torch's current
+stream).  Prints ONE JSON line: the metric nnz*samples/s, the roofline of the
+dominant kernel (HBM-bound parameter gathers) and the CPU oracle baseline
+timed on this host.
 
-Workloads:
-  plain    100,000 x 2^20 CSR, nnz/row ~ Poisson(100), lognormal(0,1) weights,
-           128 samples, plain kernel (one launch, one wavefront per row).
+Workloads (128 samples, parameters from the device Philox generator):
+  batched  (default) 64 independent CSR matrices, dim 2^20, rows per matrix
+           log-uniform in [1k, 200k], nnz/row Zipf(1.2) clipped to [1, 20000],
+           lognormal(0, 1.5) weights.  `value` is the batched persistent
+           kernel (one launch for all 64 matrices); the plain kernel (one
+           launch per matrix) is timed on the same data beside it.
+  plain    100,000 x 2^20 CSR, nnz/row ~ Poisson(100), lognormal(0, 1)
+           weights, plain kernel (one launch, one wavefront per row).
 """
 import argparse
 import ctypes
@@ -24,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "minhashcuda_amd")
 HBM_PEAK_GBS = 8000.0
 METRIC = "nnz*samples/sec (128 samples, bit-exact) batched vs plain; p50/p99 serve latency"
+SAMPLES = 128
 
 
 def log(*a):
@@ -33,13 +39,17 @@ def log(*a):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="plain", choices=["plain"])
+    p.add_argument("--steps", type=int, default=None)
+    p.add_argument("--warmup", type=int, default=None)
+    p.add_argument("--workload", default="batched", choices=["batched", "plain"])
     p.add_argument("--spl", type=int, default=0, help="samples per lane (0: default)")
+    p.add_argument("--seg", type=int, default=0, help="nonzeros per work item (0: default)")
+    p.add_argument("--wpc", type=int, default=0, help="persistent waves per CU (0: default)")
+    p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target duration of the single-thread CPU oracle sample")
+    p.add_argument("--count", type=int, default=64, help="batched: number of matrices")
     return p.parse_args()
 
 
@@ -50,26 +60,31 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(params, m, samples, target_s, gpu_hashes):
-    """Times the C oracle (this repository's CPU restatement of the hashing
-    path) on a bounded row sample: single thread, and OpenMP on every core
-    this process may use.  Also checks the sample's hashes against the GPU's."""
+def cpu_baseline(params, m, samples, target_s, gpu_hashes, what):
+    """Times this repository's C oracle of the hashing path on a bounded row
+    prefix of `m`: single thread, and OpenMP on up to 16 of the cores this
+    process may use; checks the sample's hashes against the GPU's."""
     import numpy as np
     import oracle
     rs, ln_cs, betas = params
-    # calibrate on a small prefix, then size the single-thread sample
-    n0 = min(m.shape[0], 500)
+    n0 = min(m.shape[0], 200)
+    while n0 < m.shape[0] and m[:n0].nnz * samples < 2e7:
+        n0 = min(m.shape[0], n0 * 2)
     sub = m[:n0]
     t = time.perf_counter()
     oracle.calc_csr(rs, ln_cs, betas, sub)
     rate = max(sub.nnz * samples / max(time.perf_counter() - t, 1e-6), 1.0)
-    n1 = int(min(m.shape[0], max(n0, n0 * target_s * rate / max(sub.nnz * samples, 1))))
+    want = target_s * rate / samples
+    n1 = int(np.searchsorted(m.indptr, want)) if want < m.nnz else m.shape[0]
+    n1 = max(1, min(m.shape[0], n1))
     sub = m[:n1]
     t = time.perf_counter()
     h1 = oracle.calc_csr(rs, ln_cs, betas, sub)
     t1 = time.perf_counter() - t
     threads = min(cpu_threads(), 16)
-    n2 = int(min(m.shape[0], n1 * threads))
+    want2 = sub.nnz * threads
+    n2 = int(np.searchsorted(m.indptr, want2)) if want2 < m.nnz else m.shape[0]
+    n2 = max(n1, min(m.shape[0], n2))
     sub2 = m[:n2]
     t = time.perf_counter()
     h2, used = oracle.calc_omp(rs, ln_cs, betas, sub2.data, sub2.indices, sub2.indptr, threads)
@@ -78,11 +93,56 @@ def cpu_baseline(params, m, samples, target_s, gpu_hashes):
     return {
         "value": sub2.nnz * samples / t2, "unit": "nnz*samples/s", "cores": int(used),
         "kind": "port",
-        "sample": "first %d of %d rows (%d nnz), OpenMP C oracle, %.1f s" % (n2, m.shape[0], sub2.nnz, t2),
+        "sample": "%s: first %d rows (%d nnz), OpenMP C oracle, %.1f s" % (what, n2, sub2.nnz, t2),
         "single_thread_value": sub.nnz * samples / t1,
-        "single_thread_sample": "first %d rows (%d nnz), %.1f s" % (n1, sub.nnz, t1),
+        "single_thread_sample": "%s: first %d rows (%d nnz), %.1f s" % (what, n1, sub.nnz, t1),
         "sample_bit_exact_vs_gpu": ok,
     }
+
+
+class Engine:
+    def __init__(self, dim, seed, args):
+        import libMHCUDA
+        from minhashcuda_amd import capi as capi_mod
+        self.L = libMHCUDA
+        self.capi_mod = capi_mod
+        self.capi = capi_mod.load(os.path.join(PKG, "libMHCUDA.so"))
+        self.gen = libMHCUDA.minhash_cuda_init(dim, SAMPLES, seed=seed, devices=1)
+        self.g = ctypes.c_void_p(self.gen)
+        capi_mod.check(self.capi.mhcuda_set_tuning(self.g, args.spl, args.seg, args.wpc), "tuning")
+        capi_mod.check(self.capi.mhcuda_set_timing(self.g, 1), "timing")
+
+    def tuning(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.capi.mhcuda_get_tuning(self.g, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {"samples_per_lane": a.value, "segment": b.value, "waves_per_cu": c.value}
+
+    def kernel_ms(self):
+        ms = ctypes.c_float()
+        self.capi_mod.check(self.capi.mhcuda_kernel_time(self.g, ctypes.byref(ms)), "kernel_time")
+        return ms.value
+
+    def batch(self, descs, kernel, stream):
+        self.capi_mod.check(self.capi.mhcuda_calc_batch_device(
+            self.g, len(descs), descs, kernel, ctypes.c_void_p(stream.cuda_stream)), "calc_batch")
+
+    def close(self):
+        self.L.minhash_cuda_fini(self.gen)
+
+
+def timed_steps(torch, run, engine, steps, warmup):
+    """Runs `run` warmup + steps times; returns (wall seconds of the timed
+    steps, per-step kernel milliseconds)."""
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+        kms.append(engine.kernel_ms())
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, kms
 
 
 def main():
@@ -96,86 +156,106 @@ def main():
     import torch  # before the engine: one HIP runtime per process
     sys.path.insert(0, ROOT)
     sys.path.insert(0, PKG)
-    import libMHCUDA
-    from minhashcuda_amd import capi as capi_mod
     from minhashcuda_amd import workloads
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    capi = capi_mod.load(os.path.join(PKG, "libMHCUDA.so"))
-    samples = 128
-    dim = 1 << 20
-    log("generating workload", args.workload)
-    m = workloads.plain_csr(100000, dim, 100.0, 1.0, seed=1)
-    rows, nnz = m.shape[0], m.nnz
-    gen = libMHCUDA.minhash_cuda_init(dim, samples, seed=1, devices=1)
-    if args.spl:
-        capi_mod.check(capi.mhcuda_set_tuning(ctypes.c_void_p(gen), args.spl, 0, 0), "set_tuning")
-    spl = ctypes.c_int()
-    capi.mhcuda_get_tuning(ctypes.c_void_p(gen), ctypes.byref(spl), None, None)
-    w = torch.from_numpy(m.data.astype(np.float32)).cuda()
-    c = torch.from_numpy(m.indices.astype(np.int32)).cuda()
-    r = torch.from_numpy(m.indptr.astype(np.int64).astype(np.int32)).cuda()
-    out = torch.empty((rows, samples, 2), dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
-    sptr = ctypes.c_void_p(stream.cuda_stream)
+    dim = 1 << 20
+    engine = Engine(dim, 1, args)
+    cm = engine.capi_mod
+    result = {"metric": METRIC, "unit": "nnz*samples/s", "n_gpus": 1,
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "f32", "data": "synthetic"}
 
-    def step():
-        capi_mod.check(capi.mhcuda_calc_device(ctypes.c_void_p(gen), w.data_ptr(), c.data_ptr(),
-                                               r.data_ptr(), rows, out.data_ptr(), sptr),
-                       "mhcuda_calc_device")
+    if args.workload == "plain":
+        steps = args.steps or 20
+        warmup = args.warmup if args.warmup is not None else 3
+        log("generating plain workload")
+        m = workloads.plain_csr(100000, dim, 100.0, 1.0, seed=1)
+        mats = [m]
+        tensors = [(torch.from_numpy(m.data.astype(np.float32)).cuda(),
+                    torch.from_numpy(m.indices.astype(np.int32)).cuda(),
+                    torch.from_numpy(m.indptr.astype(np.int64).astype(np.int32)).cuda())]
+        kernel = cm.KERNEL_PLAIN
+        kname = "wmh_plain_kernel"
+        desc = ("plain: 100000 x 2^20 CSR, nnz/row~Poisson(100), lognormal(0,1) weights, "
+                "128 samples, one launch (plain kernel)")
+    else:
+        steps = args.steps or 3
+        warmup = args.warmup if args.warmup is not None else 1
+        log("generating batched workload on the GPU")
+        layout = workloads.batched_layout(count=args.count, seed=1)
+        tensors = []
+        for i, n in enumerate(layout):
+            tensors.append(workloads.batched_csr_torch(n, dim, 1.5, seed=100 + i))
+        mats = None
+        kernel = cm.KERNEL_PERSISTENT
+        kname = "wmh_persistent_kernel"
+        desc = ("batched: %d CSR matrices, dim 2^20, rows/matrix log-uniform [1k,200k], "
+                "nnz/row Zipf(1.2) clipped [1,20000], lognormal(0,1.5) weights, 128 samples, "
+                "one persistent launch" % args.count)
+    lengths = [int(t[2].numel()) - 1 for t in tensors]
+    nnzs = [int(t[0].numel()) for t in tensors]
+    rows, nnz = sum(lengths), sum(nnzs)
+    outs = [torch.empty((n, SAMPLES, 2), dtype=torch.int32, device="cuda") for n in lengths]
+    descs = (cm.Matrix * len(tensors))()
+    for i, ((w, c, r), o) in enumerate(zip(tensors, outs)):
+        descs[i] = cm.Matrix(w.data_ptr(), c.data_ptr(), r.data_ptr(), lengths[i], o.data_ptr())
+    torch.cuda.synchronize()
+    log("rows %d nnz %d; running %d warmup + %d timed steps" % (rows, nnz, warmup, steps))
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    avg_ms = sum(kernel_ms) / len(kernel_ms)
-    units = nnz * samples
-    value = units * args.steps / wall
-    alg = workloads.algorithmic_bytes(nnz, rows, samples)
+    wall, kms = timed_steps(torch, lambda: engine.batch(descs, kernel, stream), engine, steps,
+                            warmup)
+    units = nnz * SAMPLES
+    avg_ms = sum(kms) / len(kms)
+    alg = workloads.algorithmic_bytes(nnz, rows, SAMPLES)
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    gpu_hashes = out.cpu().numpy().view(np.uint32)
-    result = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "nnz*samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "plain: %d x %d CSR, nnz/row~Poisson(100), lognormal(0,1) weights, "
-                        "%d samples, one launch (plain kernel)" % (rows, dim, samples),
-            "rows": rows, "dim": dim, "nnz": nnz, "samples": samples,
-            "samples_per_lane": spl.value, "parallelism": "single GPU",
-        },
-        "roofline": {
-            "bound": "hbm", "kernel": "wmh_plain_kernel", "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "algorithmic_bytes_per_launch": alg,
-            "avg_kernel_ms": avg_ms,
-        },
-    }
+    result.update({
+        "value": units * steps / wall, "steps": steps, "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps,
+        "config": dict({"workload": desc, "rows": rows, "nnz": nnz, "samples": SAMPLES,
+                        "matrices": len(tensors), "parallelism": "single GPU"},
+                       **engine.tuning()),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": alg,
+                     "avg_kernel_ms": avg_ms},
+    })
+    if args.workload == "batched" and not args.no_plain:
+        ref_out = [o.clone() for o in outs]
+        for o in outs:
+            o.fill_(0)
+        pwall, pkms = timed_steps(torch, lambda: engine.batch(descs, cm.KERNEL_PLAIN, stream),
+                                  engine, steps, warmup)
+        same = all(torch.equal(a, b) for a, b in zip(ref_out, outs))
+        result["plain"] = {"value": units * steps / pwall, "ms_per_step": pwall * 1e3 / steps,
+                           "avg_kernel_ms": sum(pkms) / len(pkms),
+                           "launches_per_step": len(tensors),
+                           "identical_to_batched": bool(same)}
+        result["batched_over_plain"] = result["value"] / result["plain"]["value"]
+        del ref_out
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        params = libMHCUDA.minhash_cuda_retrieve_vars(gen)
-        result["cpu_baseline"] = cpu_baseline(params, m, samples, args.cpu_seconds, gpu_hashes)
-    libMHCUDA.minhash_cuda_fini(gen)
+        params = engine.L.minhash_cuda_retrieve_vars(engine.gen)
+        if mats is None:
+            # the CPU sample: the first rows of the first matrix, copied back
+            w, c, r = tensors[0]
+            rp = r.cpu().numpy().view(np.uint32).astype(np.int64)
+            cut = int(np.searchsorted(rp, 60_000_000))
+            cut = max(1, min(lengths[0], cut))
+            import scipy.sparse as sp
+            m0 = sp.csr_matrix((w[:rp[cut]].cpu().numpy(),
+                                c[:rp[cut]].cpu().numpy().view(np.uint32),
+                                rp[:cut + 1]), shape=(cut, dim))
+            gh = outs[0][:cut].cpu().numpy().view(np.uint32)
+            result["cpu_baseline"] = cpu_baseline(params, m0, SAMPLES, args.cpu_seconds, gh,
+                                                  "matrix 0 of the batch")
+        else:
+            gh = outs[0].cpu().numpy().view(np.uint32)
+            result["cpu_baseline"] = cpu_baseline(params, mats[0], SAMPLES, args.cpu_seconds,
+                                                  gh, "plain workload")
+    engine.close()
     print(json.dumps(result), flush=True)
 
 

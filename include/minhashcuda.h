@@ -136,6 +136,14 @@ MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen,
                                int *samples_per_lane, int *segment,
                                int *waves_per_cu);
 
+/* Enables (1) or disables (0) HIP-event timing of the hashing kernel(s) of
+ * each call: the persistent kernel alone, or the span of the plain launches. */
+MHCUDAResult mhcuda_set_timing(MinhashCudaGenerator *gen, int enable);
+
+/* Milliseconds taken by the timed kernel(s) of the last hashing call (waits
+ * for them to finish); mhcudaInvalidArguments if that call was not timed. */
+MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *gen, float *ms);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

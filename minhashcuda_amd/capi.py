@@ -22,7 +22,7 @@ EXPORTS = (
     "mhcuda_init", "mhcuda_get_parameters", "mhcuda_retrieve_random_vars",
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
-    "mhcuda_set_tuning", "mhcuda_get_tuning",
+    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_timing", "mhcuda_kernel_time",
 )
 
 
@@ -65,6 +65,10 @@ def load(path):
     L.mhcuda_get_tuning.restype = i32
     L.mhcuda_get_tuning.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                     ctypes.POINTER(i32)]
+    L.mhcuda_set_timing.restype = i32
+    L.mhcuda_set_timing.argtypes = [vp, i32]
+    L.mhcuda_kernel_time.restype = i32
+    L.mhcuda_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     return L
 
 

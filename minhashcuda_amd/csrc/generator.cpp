@@ -78,11 +78,25 @@ struct MinhashCudaGenerator_ {
   std::mutex mu;
   // grow-only caches, reference minhashcuda.cc:27-31
   DevBuf weights, cols, rows, hashes, staging, plan, scratch;
-  wmh::Tuning tuning{2, 2048, 8};
+  wmh::Tuning tuning{2, 2048, 16};
   int kernel = mhcudaKernelAuto;
+  // pinned staging of the batch descriptors; `desc_copied` marks the end of
+  // the last copy out of it so that it is not overwritten too early
+  wmh::MatrixDesc *desc_host = nullptr;
+  size_t desc_host_cap = 0;
+  hipEvent_t desc_copied = nullptr;
+  // optional timing of the hashing kernel(s) of the last call
+  bool timing = false;
+  bool timed = false;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 
   ~MinhashCudaGenerator_() {
+    if (ev_begin) (void)hipEventDestroy(ev_begin);
+    if (ev_end) (void)hipEventDestroy(ev_end);
     if (device >= 0) (void)hipSetDevice(device);
+    if (desc_copied) (void)hipEventSynchronize(desc_copied);
+    if (desc_copied) (void)hipEventDestroy(desc_copied);
+    if (desc_host) (void)hipHostFree(desc_host);
     if (stream) (void)hipStreamDestroy(stream);
     if (params) (void)hipFree(params);
   }
@@ -148,6 +162,47 @@ uint32_t relayout_chunk(uint32_t dim, uint32_t samples) {
 
 namespace wmh {
 
+// Enqueues the persistent batched path on `s`: descriptors go through the
+// pinned staging buffer into device memory next to the control block.
+static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
+                                    hipStream_t s) {
+  int verbosity = gen->verbosity;
+  const uint32_t count = (uint32_t)mats.size();
+  uint64_t chunks = 0;
+  for (MatrixDesc &m : mats) {
+    m.item_begin = m.n_items = 0;
+    m.chunk_begin = (uint32_t)chunks;
+    chunks += (m.length + 63u) / 64u;
+  }
+  if (chunks >= (1ull << 32)) return mhcudaInvalidArguments;
+  const size_t desc_bytes = sizeof(MatrixDesc) * count;
+  const size_t ctl_off = (desc_bytes + 255) & ~(size_t)255;
+  HIPCH(gen->plan.reserve(ctl_off + sizeof(BatchCtl)), mhcudaMemoryAllocationFailure);
+  if (!gen->desc_copied)
+    HIPCH(hipEventCreateWithFlags(&gen->desc_copied, hipEventDisableTiming), mhcudaRuntimeError);
+  HIPCH(hipEventSynchronize(gen->desc_copied), mhcudaRuntimeError);
+  if (gen->desc_host_cap < count) {
+    if (gen->desc_host) (void)hipHostFree(gen->desc_host);
+    gen->desc_host = nullptr;
+    gen->desc_host_cap = 0;
+    HIPCH(hipHostMalloc((void **)&gen->desc_host, desc_bytes, hipHostMallocDefault),
+          mhcudaMemoryAllocationFailure);
+    gen->desc_host_cap = count;
+  }
+  std::memcpy(gen->desc_host, mats.data(), desc_bytes);
+  char *dev = (char *)gen->plan.ptr;
+  HIPCH(hipMemcpyAsync(dev, gen->desc_host, desc_bytes, hipMemcpyHostToDevice, s),
+        mhcudaMemoryCopyError);
+  HIPCH(hipEventRecord(gen->desc_copied, s), mhcudaRuntimeError);
+  HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
+                          (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), gen->tuning,
+                          gen->num_cus, s, gen->timing ? gen->ev_begin : nullptr,
+                          gen->timing ? gen->ev_end : nullptr),
+        mhcudaRuntimeError);
+  gen->timed = gen->timing;
+  return mhcudaSuccess;
+}
+
 // Launches the chosen hashing kernel over device-resident matrices on `s`.
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen,
                                std::vector<MatrixDesc> &mats, int kernel, hipStream_t s) {
@@ -158,14 +213,17 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen,
     return mhcudaInvalidArguments;
   }
   if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
-  if (kernel == mhcudaKernelAuto || kernel == mhcudaKernelPersistent) {
-    kernel = mhcudaKernelPlain;  // the persistent path is added separately
-  }
+  if (kernel == mhcudaKernelAuto) kernel = mhcudaKernelPersistent;
+  if (kernel == mhcudaKernelPersistent) return hash_persistent(gen, mats, s);
+  if (kernel != mhcudaKernelPlain) return mhcudaInvalidArguments;
+  if (gen->timing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
   for (const MatrixDesc &m : mats) {
     if (m.length == 0) continue;
     HIPCH(launch_plain(gen->params, gen->dim, gen->samples, m, gen->tuning.spl, s),
           mhcudaRuntimeError);
   }
+  if (gen->timing) HIPCH(hipEventRecord(gen->ev_end, s), mhcudaRuntimeError);
+  gen->timed = gen->timing;
   return mhcudaSuccess;
 }
 
@@ -329,7 +387,7 @@ MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t
   std::vector<wmh::MatrixDesc> descs(count);
   for (uint32_t m = 0; m < count; m++) {
     descs[m] = wmh::MatrixDesc{mats[m].weights, mats[m].cols, mats[m].rows, mats[m].output,
-                               mats[m].length, 0u, 0u, 0u};
+                               mats[m].length, 0u, 0u, 0u, 0u, 0u};
   }
   MHCUDAResult r = wmh::hash_device_batch(gen, descs, kernel, s);
   if (r != mhcudaSuccess) return r;
@@ -347,7 +405,9 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     const MHCUDAMatrix &x = mats[m];
     if (x.length == 0) continue;
     if (!x.weights || !x.cols || !x.rows || !x.output) return mhcudaInvalidArguments;
-    if (x.rows[x.length] < x.rows[0]) return mhcudaInvalidArguments;
+    for (uint32_t r = 0; r < x.length; r++) {
+      if (x.rows[r + 1] < x.rows[r]) return mhcudaInvalidArguments;  // malformed indptr
+    }
     total_nnz += x.rows[x.length] - x.rows[0];
     total_rows += (uint64_t)x.length + 1;
     total_out += (uint64_t)x.length * gen->samples * 2;
@@ -370,7 +430,7 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
            *dh = (uint32_t *)gen->hashes.ptr;
   for (uint32_t m = 0; m < count; m++) {
     const MHCUDAMatrix &x = mats[m];
-    dev[m] = wmh::MatrixDesc{nullptr, nullptr, nullptr, nullptr, 0u, 0u, 0u, 0u};
+    dev[m] = wmh::MatrixDesc{nullptr, nullptr, nullptr, nullptr, 0u, 0u, 0u, 0u, 0u, 0u};
     if (x.length == 0) continue;
     const uint64_t nnz = x.rows[x.length] - x.rows[0];
     // the device copy of the matrix starts at CSR position rows[0]
@@ -416,6 +476,31 @@ MHCUDAResult mhcuda_set_tuning(MinhashCudaGenerator *gen, int samples_per_lane,
   if (samples_per_lane) gen->tuning.spl = samples_per_lane;
   if (segment) gen->tuning.seg = segment;
   if (waves_per_cu) gen->tuning.waves_per_cu = waves_per_cu;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_set_timing(MinhashCudaGenerator *gen, int enable) {
+  if (!gen) return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  int verbosity = gen->verbosity;
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  if (enable && !gen->ev_begin) {
+    HIPCH(hipEventCreate(&gen->ev_begin), mhcudaRuntimeError);
+    HIPCH(hipEventCreate(&gen->ev_end), mhcudaRuntimeError);
+  }
+  gen->timing = enable != 0;
+  gen->timed = false;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *cgen, float *ms) {
+  if (!cgen || !ms) return mhcudaInvalidArguments;
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  std::lock_guard<std::mutex> lock(gen->mu);
+  int verbosity = gen->verbosity;
+  if (!gen->timed) return mhcudaInvalidArguments;
+  HIPCH(hipEventSynchronize(gen->ev_end), mhcudaRuntimeError);
+  HIPCH(hipEventElapsedTime(ms, gen->ev_begin, gen->ev_end), mhcudaRuntimeError);
   return mhcudaSuccess;
 }
 

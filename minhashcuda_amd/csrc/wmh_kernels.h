@@ -23,8 +23,20 @@ struct MatrixDesc {
   uint32_t *output;       // [length][samples][2]
   uint32_t length;
   uint32_t base;
-  uint32_t item_begin;    // first work item of this matrix (persistent kernel)
-  uint32_t n_items;       // work items covering this matrix
+  uint32_t item_begin;    // first work item of this matrix (set on the device)
+  uint32_t n_items;       // work items covering this matrix (set on the device)
+  uint32_t chunk_begin;   // first 64-row chunk of this matrix in the row scans
+  uint32_t pad_;
+};
+
+// Control block of one persistent launch (device memory).
+struct BatchCtl {
+  uint32_t next_unit;     // work queue head
+  uint32_t total_units;   // items * sample tiles
+  uint32_t total_items;
+  uint32_t ntiles;
+  uint32_t invalid;       // nonzero: some indptr decreases; nothing is hashed
+  uint32_t pad_[3];
 };
 
 // Kernel tuning knobs (chosen by the autotuner).
@@ -51,5 +63,17 @@ hipError_t launch_gather_params(const float *params, uint32_t j0, uint32_t ncols
 // (row, sample tile).
 hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
                         const MatrixDesc &m, int spl, hipStream_t stream);
+
+// Batched hashing of `count` matrices in one persistent launch (plus the
+// plan, init and fix-up passes).  `mats` and `ctl` are device memory; the
+// host fills every MatrixDesc field except item_begin / n_items, and
+// chunk_begin must be the exclusive prefix of ceil(length / 64).
+// `ev_begin` / `ev_end` (may be null) are recorded right around the
+// persistent hashing kernel.
+hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
+                             MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
+                             BatchCtl *ctl, const Tuning &tuning, int num_cus,
+                             hipStream_t stream, hipEvent_t ev_begin = nullptr,
+                             hipEvent_t ev_end = nullptr);
 
 }  // namespace wmh

@@ -21,6 +21,8 @@
 // contraction; the kernel is bound by HBM bandwidth on those gathers.
 #include <float.h>
 
+#include <algorithm>
+
 #include "wmh_kernels.h"
 #include "wmh_math.h"
 
@@ -308,12 +310,16 @@ __global__ __launch_bounds__(kBlock) void wmh_plain_kernel(
     uint32_t length, uint32_t base0, uint32_t *__restrict__ output) {
   const uint64_t total = (uint64_t)length * ntiles;
   const uint32_t lane = lane_id();
+  // CSR positions are clamped to [rows[0], rows[length]], so a malformed
+  // indptr can give wrong hashes but never reads outside the arrays
+  const uint32_t first = rows[0] - base0;
+  const uint32_t last = max(first, rows[length] - base0);
   for (uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
        w < total; w += (uint64_t)gridDim.x * kWavesPerBlock) {
     const uint32_t row = uniform((uint32_t)(w / ntiles));
     const uint32_t tile = uniform((uint32_t)(w - (uint64_t)row * ntiles));
-    const uint32_t begin = uniform(rows[row] - base0);
-    const uint32_t end = uniform(rows[row + 1] - base0);
+    const uint32_t begin = uniform(min(max(rows[row] - base0, first), last));
+    const uint32_t end = uniform(min(max(rows[row + 1] - base0, first), last));
     const uint32_t k0 = tile * (kWave * V) + lane * V;
     const uint32_t koff = (k0 < samples) ? k0 : 0u;
     const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
@@ -321,6 +327,320 @@ __global__ __launch_bounds__(kBlock) void wmh_plain_kernel(
     acc.init();
     hash_range<V, 16 / V>(params, dim, 12ull * sp, lo, weights, cols, begin, end, acc);
     if (k0 < samples) write_hashes<V>(output, row, samples, k0, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched, nnz-balanced persistent hashing.
+//
+// The nonzeros of each matrix are cut into items of `seg` consecutive CSR
+// positions (the last item of a matrix may be shorter); a work unit is an
+// (item, sample tile) pair.  Waves of a fixed-size grid pull units from an
+// atomic queue until it is drained.  Inside an item a wave streams the
+// nonzeros in chunks of 64 regardless of row boundaries, so many short rows
+// share one wavefront, and flushes a row's argmins whenever the row ends.
+//
+// A row that crosses an item boundary ("split") is merged across items with a
+// 64-bit atomicMin on key = order(ln_a) << 32 | position, written into the
+// row's own 8-byte output slots: the key orders by ln_a, then by CSR position,
+// which is exactly the strict-< first-minimum of the sequential walk.  A
+// fix-up pass then turns each winning key back into (column, t), recomputing
+// t for the winning position.  Empty and split rows are pre-filled with all
+// ones (the sentinel pair == the "no key" value) by an init pass.
+//
+// Launch sequence: plan (item prefix over matrices, queue reset) -> init ->
+// persistent -> fixup.  No memsets, so the sequence can be graph-captured.
+
+// Maps a binary32 value that can be selected (not NaN, < FLT_MAX) to an
+// unsigned integer with the same order; -0 and +0 share a key.
+__device__ __forceinline__ uint32_t order_key(float f) {
+  const uint32_t u = wmh_f2u(f == 0.0f ? 0.0f : f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+constexpr unsigned long long kNoKey = ~0ull;
+
+// The row containing CSR position a: the largest r in [0, length] with
+// rows[r] <= a, given rows[0] <= a < rows[length].  64-ary search: one
+// wavefront-wide load and one ballot per level.
+__device__ __forceinline__ uint32_t find_row(const uint32_t *__restrict__ rows,
+                                             uint32_t length, uint32_t a) {
+  const uint32_t lane = lane_id();
+  uint32_t lo = 0, hi = length;  // rows[lo] <= a; the answer is in [lo, hi]
+  while (hi - lo >= (uint32_t)kWave) {
+    const uint32_t step = (hi - lo + kWave) / kWave;  // ceil((hi - lo + 1) / 64)
+    const uint32_t idx = lo + lane * step;
+    const bool le = idx <= hi && rows[idx] <= a;
+    const unsigned long long mask = __ballot(le);  // lane 0 is always set
+    lo = uniform(lo + (uint32_t)(63 - __builtin_clzll(mask)) * step);
+    hi = uniform(min(hi, lo + step - 1u));
+  }
+  const uint32_t idx = lo + lane;
+  const bool le = idx <= hi && rows[idx] <= a;
+  const unsigned long long mask = __ballot(le);
+  return uniform(lo + (uint32_t)(63 - __builtin_clzll(mask)));
+}
+
+// Flushes the argmins of one row: final hashes for a row that lies inside
+// the item, keyed atomic minima for a split row.
+template <int V>
+__device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_t row,
+                                          uint32_t samples, uint32_t k0, bool split,
+                                          const Argmin<V> &acc) {
+  if (k0 >= samples) return;
+  if (split) {
+    unsigned long long *slot =
+        reinterpret_cast<unsigned long long *>(output) + ((uint64_t)row * samples + k0);
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      if (k0 + v < samples && acc.j[v] != WMH_SENTINEL) {
+        const unsigned long long key =
+            ((unsigned long long)order_key(acc.best[v]) << 32) | acc.j[v];
+        atomicMin(slot + v, key);
+      }
+    }
+  } else {
+    write_hashes<V>(output, row, samples, k0, acc);
+  }
+}
+
+// Hashes CSR positions [a, b) of matrix M (one item) for the lane's samples.
+// In a split row the argmin tracks the CSR position instead of the column.
+template <int V, int U>
+__device__ __forceinline__ void hash_item(const float *__restrict__ params, uint32_t dim,
+                                          uint32_t sp, uint32_t samples, uint32_t k0,
+                                          LaneOffsets lo, const MatrixDesc &M,
+                                          uint32_t a, uint32_t b) {
+  const uint32_t lane = lane_id();
+  const char *pbase = reinterpret_cast<const char *>(params);
+  const uint64_t block_bytes = 12ull * sp;
+  const uint32_t *__restrict__ rows = M.rows;
+  const uint32_t *__restrict__ cols = M.cols - M.base;
+  const float *__restrict__ weights = M.weights - M.base;
+  uint32_t r = find_row(rows, M.length, a);
+  uint32_t row_start = uniform(rows[r]);
+  // rows[rp_base + lane]: the next 64 row ends, read with one vector load
+  uint32_t rp_base = r + 1;
+  uint32_t rp = (rp_base + lane <= M.length) ? rows[rp_base + lane] : 0xffffffffu;
+  uint32_t rp_idx = 0;
+  uint32_t row_end = readlane_u32(rp, 0);
+  bool split = row_start < a || row_end > b;
+  Argmin<V> acc;
+  acc.init();
+
+  // Called after position p_next - 1 ended row r and p_next < b: moves to the
+  // next non-empty row (it contains p_next).
+  auto next_row = [&]() {
+    do {
+      r++;
+      row_start = row_end;
+      if (++rp_idx == (uint32_t)kWave) {
+        rp_base += kWave;
+        rp = (rp_base + lane <= M.length) ? rows[rp_base + lane] : 0xffffffffu;
+        rp_idx = 0;
+      }
+      row_end = readlane_u32(rp, rp_idx);
+    } while (row_end == row_start);
+    split = row_end > b;
+    acc.init();
+  };
+
+  for (uint32_t cb = a; cb < b; cb += kWave) {
+    const uint32_t n = uniform(min((uint32_t)kWave, b - cb));
+    uint32_t c = WMH_SENTINEL;
+    float lw = 0.0f;
+    if (lane < n) {
+      c = cols[cb + lane];
+      lw = wmh_logf(weights[cb + lane]);
+    }
+    uint32_t i = 0;
+    for (; i + U <= n; i += U) {
+      uint32_t j[U];
+      float lwu[U];
+      float rr[U][V], beta[U][V], lc[U][V];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        j[u] = readlane_u32(c, i + u);
+        lwu[u] = readlane_f32(lw, i + u);
+        const char *blk = pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes;
+        load_v<V>(at(blk, lo.r), rr[u]);
+        load_v<V>(at(blk, lo.beta), beta[u]);
+        load_v<V>(at(blk, lo.lc), lc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint32_t p = cb + i + u;
+        fold<V>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u], j[u] < dim, acc);
+        if (p + 1 == row_end) {
+          flush_row<V>(M.output, r, samples, k0, split, acc);
+          if (p + 1 < b) next_row();
+        }
+      }
+    }
+    for (; i < n; i++) {
+      const uint32_t j = readlane_u32(c, i);
+      const float lwi = readlane_f32(lw, i);
+      const char *blk = pbase + (uint64_t)(j < dim ? j : 0u) * block_bytes;
+      float rr[V], beta[V], lc[V];
+      load_v<V>(at(blk, lo.r), rr);
+      load_v<V>(at(blk, lo.beta), beta);
+      load_v<V>(at(blk, lo.lc), lc);
+      const uint32_t p = cb + i;
+      fold<V>(rr, beta, lc, split ? p : j, lwi, j < dim, acc);
+      if (p + 1 == row_end) {
+        flush_row<V>(M.output, r, samples, k0, split, acc);
+        if (p + 1 < b) next_row();
+      }
+    }
+  }
+  if (row_end > b) flush_row<V>(M.output, r, samples, k0, true, acc);  // continues in a later item
+}
+
+// Matrix whose range of `field` values contains x: the largest m with
+// mats[m].field <= x (mats sorted by field).
+template <uint32_t MatrixDesc::*Field>
+__device__ __forceinline__ uint32_t find_matrix(const MatrixDesc *__restrict__ mats,
+                                                uint32_t count, uint32_t x) {
+  uint32_t lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (mats[mid].*Field <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return uniform(lo);
+}
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
+    const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
+    const MatrixDesc *__restrict__ mats, uint32_t count, uint32_t seg,
+    BatchCtl *__restrict__ ctl) {
+  const uint32_t lane = lane_id();
+  if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
+  const uint32_t total = uniform(ctl->total_units);
+  const uint32_t ntiles = uniform(ctl->ntiles);
+  for (;;) {
+    uint32_t u = 0;
+    if (lane == 0) u = atomicAdd(&ctl->next_unit, 1u);
+    u = readlane_u32(u, 0);
+    if (u >= total) break;  // every wave reaches this once the queue is drained
+    const uint32_t item = u / ntiles;
+    const uint32_t tile = u - item * ntiles;
+    const uint32_t m = find_matrix<&MatrixDesc::item_begin>(mats, count, item);
+    const MatrixDesc M = mats[m];
+    const uint32_t rows0 = uniform(M.rows[0]);
+    const uint32_t rend = uniform(M.rows[M.length]);
+    const uint32_t a = rows0 + (item - M.item_begin) * seg;
+    const uint32_t b = a + min(seg, rend - a);
+    const uint32_t k0 = tile * (kWave * V) + lane * V;
+    const uint32_t koff = (k0 < samples) ? k0 : 0u;
+    const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
+    hash_item<V, 16 / V>(params, dim, sp, samples, k0, lo, M, a, b);
+  }
+}
+
+// Item prefix over the matrices (one workgroup) and reset of the work queue.
+__global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restrict__ mats,
+                                                            uint32_t count, uint32_t seg,
+                                                            uint32_t ntiles,
+                                                            BatchCtl *__restrict__ ctl) {
+  __shared__ uint32_t part[kBlock];
+  __shared__ uint32_t carry;
+  const uint32_t tid = threadIdx.x;
+  __shared__ uint32_t bad;
+  if (tid == 0) {
+    carry = 0;
+    bad = 0;
+  }
+  __syncthreads();
+  for (uint32_t base = 0; base < count; base += kBlock) {
+    const uint32_t m = base + tid;
+    uint32_t n = 0;
+    if (m < count && mats[m].length > 0) {
+      const uint32_t first = mats[m].rows[0], last = mats[m].rows[mats[m].length];
+      if (last < first) {
+        bad = 1;
+      } else {
+        const uint32_t nnz = last - first;
+        n = nnz / seg + (nnz % seg != 0u);
+      }
+    }
+    part[tid] = n;
+    __syncthreads();
+    for (uint32_t off = 1; off < (uint32_t)kBlock; off <<= 1) {
+      const uint32_t v = tid >= off ? part[tid - off] : 0u;
+      __syncthreads();
+      part[tid] += v;
+      __syncthreads();
+    }
+    if (m < count) {
+      mats[m].item_begin = carry + part[tid] - n;
+      mats[m].n_items = n;
+    }
+    __syncthreads();
+    if (tid == kBlock - 1) carry += part[kBlock - 1];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ctl->next_unit = 0;
+    ctl->invalid = bad;
+    ctl->total_items = carry;
+    ctl->total_units = carry * ntiles;
+    ctl->ntiles = ntiles;
+  }
+}
+
+// Row scan over every matrix, 64 rows per wavefront.
+//   kInit:  empty rows and split rows get all-ones in every sample slot;
+//   kFixup: each split row's winning keys become (column, t).
+enum RowPass { kInit = 0, kFixup = 1 };
+
+template <int PASS>
+__global__ __launch_bounds__(kBlock) void batch_rows_kernel(
+    const float *__restrict__ params, uint32_t samples, uint32_t sp,
+    const MatrixDesc *__restrict__ mats, uint32_t count, uint32_t seg,
+    uint32_t total_chunks, BatchCtl *__restrict__ ctl) {
+  WMH_NO_CONTRACT
+  const uint32_t lane = lane_id();
+  if (PASS == kFixup && uniform(ctl->invalid)) return;
+  for (uint32_t w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); w < total_chunks;
+       w += gridDim.x * kWavesPerBlock) {
+    const uint32_t wv = uniform(w);
+    const uint32_t m = find_matrix<&MatrixDesc::chunk_begin>(mats, count, wv);
+    const MatrixDesc M = mats[m];
+    const uint32_t r0 = (wv - M.chunk_begin) * kWave;
+    const uint32_t row = r0 + lane;
+    bool flag = false;
+    if (row < M.length) {
+      const uint32_t rows0 = M.rows[0];
+      const uint32_t s = M.rows[row], e = M.rows[row + 1];
+      if (PASS == kInit && (e < s || s < rows0)) atomicOr(&ctl->invalid, 1u);
+      const bool empty = s == e;
+      const bool split = !empty && (s - rows0) / seg != (e - 1u - rows0) / seg;
+      flag = PASS == kInit ? (empty || split) : split;
+    }
+    unsigned long long mask = __ballot(flag);
+    while (mask) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(mask);
+      mask &= mask - 1;
+      const uint32_t rr = uniform(r0 + l);
+      unsigned long long *slot =
+          reinterpret_cast<unsigned long long *>(M.output) + (uint64_t)rr * samples;
+      for (uint32_t k = lane; k < samples; k += kWave) {
+        if (PASS == kInit) {
+          slot[k] = kNoKey;
+        } else {
+          const unsigned long long key = slot[k];
+          if (key == kNoKey) continue;
+          const uint32_t pos = (uint32_t)key - M.base;
+          const uint32_t j = M.cols[pos];
+          const float lw = wmh_logf(M.weights[pos]);
+          const float *blk = params + (uint64_t)j * 3u * sp;
+          const float t = __builtin_floorf(lw / blk[k] + blk[sp + k]);
+          reinterpret_cast<uint2 *>(slot)[k] = make_uint2(j, wmh_t_bits(t));
+        }
+      }
+    }
   }
 }
 
@@ -387,6 +707,44 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
     default:
       return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
+                             MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
+                             BatchCtl *ctl, const Tuning &tuning, int num_cus,
+                             hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  const uint32_t sp = padded_samples(samples);
+  const int spl = tuning.spl;
+  if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
+  if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
+  const uint32_t seg = (uint32_t)tuning.seg;
+  const uint32_t per_wave = (uint32_t)(kWave * spl);
+  const uint32_t ntiles = (samples + per_wave - 1) / per_wave;
+  hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(kBlock), 0, stream, mats, count, seg,
+                     ntiles, ctl);
+  const unsigned row_grid = grid_for(total_chunks);
+  hipLaunchKernelGGL(batch_rows_kernel<kInit>, dim3(row_grid), dim3(kBlock), 0, stream,
+                     params, samples, sp, mats, count, seg, total_chunks, ctl);
+  const unsigned grid = (unsigned)std::max(1, num_cus * tuning.waves_per_cu / kWavesPerBlock);
+  if (ev_begin) (void)hipEventRecord(ev_begin, stream);
+  switch (spl) {
+    case 1:
+      hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
+                         params, dim, samples, sp, mats, count, seg, ctl);
+      break;
+    case 2:
+      hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
+                         params, dim, samples, sp, mats, count, seg, ctl);
+      break;
+    default:
+      hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
+                         params, dim, samples, sp, mats, count, seg, ctl);
+      break;
+  }
+  if (ev_end) (void)hipEventRecord(ev_end, stream);
+  hipLaunchKernelGGL(batch_rows_kernel<kFixup>, dim3(row_grid), dim3(kBlock), 0, stream,
+                     params, samples, sp, mats, count, seg, total_chunks, ctl);
   return hipGetLastError();
 }
 

@@ -1,0 +1,120 @@
+"""GPU parity of the batched persistent path (and the plain path on batches).
+
+The uneven batch is scaled down so the oracle can check every hash: Zipf(1.2)
+row lengths clipped to [1, 20000], log-uniform row counts, empty rows and an
+empty matrix, lognormal(0, 1.5) weights.  Each row's result must equal the
+oracle's bit for bit whatever the work-item size, lane width, queue size or
+kernel -- which is also the "alone == in the batch" property, since the
+oracle hashes every row on its own.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from minhashcuda_amd import capi as capi_mod
+from minhashcuda_amd import workloads
+
+pytestmark = pytest.mark.gpu
+PLAIN, PERSISTENT = capi_mod.KERNEL_PLAIN, capi_mod.KERNEL_PERSISTENT
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def ref_hashes(params, m):
+    return oracle.calc_omp(*params, m.data, m.indices, m.indptr, 16)[0]
+
+
+@pytest.fixture(scope="module")
+def batch():
+    layout = workloads.batched_layout(count=6, min_rows=20, max_rows=600, zipf_s=1.2,
+                                      clip=20000, seed=3)
+    mats = [workloads.batched_csr(n, dim=1 << 14, sigma=1.5, seed=i) for i, n in enumerate(layout)]
+    # an all-empty matrix, an empty row at the start and a run of empty rows
+    mats.append(sp.csr_matrix((7, 1 << 14), dtype=np.float32))
+    m = mats[0].tolil()
+    m[0, :] = 0
+    m[5:9, :] = 0
+    mats[0] = m.tocsr()
+    mats[0].eliminate_zeros()
+    return mats
+
+
+@pytest.fixture(scope="module")
+def expected(batch, libmhcuda, gpu_available):
+    gen = libmhcuda.minhash_cuda_init(1 << 14, 128, seed=21)
+    params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+    libmhcuda.minhash_cuda_fini(gen)
+    return params, [ref_hashes(params, m) for m in batch]
+
+
+@pytest.mark.parametrize("seg,spl,wpc", [(64, 2, 16), (256, 1, 8), (2048, 2, 16),
+                                         (4096, 4, 24), (128, 2, 1)])
+def test_persistent_matches_oracle(libmhcuda, capi, batch, expected, seg, spl, wpc):
+    say("seg", seg, "spl", spl, "waves/CU", wpc)
+    params, ref = expected
+    gen = libmhcuda.minhash_cuda_init(1 << 14, 128, deferred=True)
+    libmhcuda.minhash_cuda_assign_vars(gen, *params)
+    assert capi.mhcuda_set_tuning(ctypes.c_void_p(gen), spl, seg, wpc) == 0
+    got = libmhcuda.minhash_cuda_calc_batch(gen, batch, kernel=PERSISTENT)
+    # a second call reuses every buffer and must give the same answer
+    again = libmhcuda.minhash_cuda_calc_batch(gen, batch[::-1], kernel=PERSISTENT)[::-1]
+    libmhcuda.minhash_cuda_fini(gen)
+    for i, (g, a, r) in enumerate(zip(got, again, ref)):
+        assert np.array_equal(g, r), "matrix %d" % i
+        assert np.array_equal(a, r), "matrix %d (second call)" % i
+
+
+def test_plain_batch_matches_oracle(libmhcuda, batch, expected):
+    params, ref = expected
+    gen = libmhcuda.minhash_cuda_init(1 << 14, 128, deferred=True)
+    libmhcuda.minhash_cuda_assign_vars(gen, *params)
+    got = libmhcuda.minhash_cuda_calc_batch(gen, batch, kernel=PLAIN)
+    libmhcuda.minhash_cuda_fini(gen)
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
+
+
+@pytest.mark.parametrize("samples", [2, 100, 256])
+def test_persistent_sample_counts(libmhcuda, capi, batch, samples):
+    say("samples", samples)
+    gen = libmhcuda.minhash_cuda_init(1 << 14, samples, seed=samples)
+    params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+    assert capi.mhcuda_set_tuning(ctypes.c_void_p(gen), 0, 512, 0) == 0
+    got = libmhcuda.minhash_cuda_calc_batch(gen, batch[:3], kernel=PERSISTENT)
+    libmhcuda.minhash_cuda_fini(gen)
+    for g, m in zip(got, batch[:3]):
+        assert np.array_equal(g, ref_hashes(params, m))
+
+
+def test_single_matrix_default_path(libmhcuda, capi, batch, expected):
+    """minhash_cuda_calc (the reference entry point) on a matrix with
+    20000-nonzero rows, split across many work items."""
+    params, ref = expected
+    gen = libmhcuda.minhash_cuda_init(1 << 14, 128, deferred=True)
+    libmhcuda.minhash_cuda_assign_vars(gen, *params)
+    assert capi.mhcuda_set_tuning(ctypes.c_void_p(gen), 0, 1024, 0) == 0
+    big = int(np.argmax([m.nnz for m in batch]))
+    h = libmhcuda.minhash_cuda_calc(gen, batch[big])
+    h2 = libmhcuda.minhash_cuda_calc(gen, batch[big], row_start=3, row_finish=batch[big].shape[0] - 2)
+    libmhcuda.minhash_cuda_fini(gen)
+    assert np.array_equal(h, ref[big])
+    assert np.array_equal(h2, ref[big][3:-2])
+
+
+def test_host_rejects_malformed_indptr(libmhcuda, capi):
+    gen = libmhcuda.minhash_cuda_init(16, 8)
+    w = np.ones(4, np.float32)
+    c = np.arange(4, dtype=np.uint32)
+    rows = np.array([0, 3, 1, 4], np.uint32)  # decreasing
+    out = np.zeros((3, 8, 2), np.uint32)
+    rc = capi.mhcuda_calc(ctypes.c_void_p(gen), w.ctypes.data, c.ctypes.data, rows.ctypes.data,
+                          3, out.ctypes.data)
+    libmhcuda.minhash_cuda_fini(gen)
+    assert rc == 1
