@@ -41,7 +41,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", default="batched", choices=["batched", "plain"])
+    p.add_argument("--workload", default="batched", choices=["batched", "plain", "copy"],
+                   help="copy: a plain 4 GiB device-to-device copy (calibrates counters)")
     p.add_argument("--spl", type=int, default=0, help="samples per lane (0: default)")
     p.add_argument("--seg", type=int, default=0, help="nonzeros per work item (0: default)")
     p.add_argument("--wpc", type=int, default=0, help="persistent waves per CU (0: default)")
@@ -161,6 +162,16 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    if args.workload == "copy":
+        n = 1 << 30  # 4 GiB of float32
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"workload": "copy", "bytes_read_per_copy": 4 * n,
+                          "bytes_written_per_copy": 4 * n, "copies": 3}), flush=True)
+        return
     dim = 1 << 20
     engine = Engine(dim, 1, args)
     cm = engine.capi_mod

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cinttypes>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -77,7 +78,13 @@ struct MinhashCudaGenerator_ {
   hipStream_t stream = nullptr;
   std::mutex mu;
   // grow-only caches, reference minhashcuda.cc:27-31
-  DevBuf weights, cols, rows, hashes, staging, plan, scratch;
+  DevBuf staging, plan;
+  // host-memory calls: two device buffer sets used alternately, their own
+  // upload and download streams, and the size limits of one piece
+  wmh::PieceBuffers pieces[2];
+  hipStream_t copy_stream = nullptr, d2h_stream = nullptr;
+  uint64_t piece_nnz = 1ull << 28;
+  uint64_t piece_rows = 1ull << 20;
   wmh::Tuning tuning{2, 2048, 16};
   int kernel = mhcudaKernelAuto;
   // pinned staging of the batch descriptors; `desc_copied` marks the end of
@@ -97,7 +104,13 @@ struct MinhashCudaGenerator_ {
     if (desc_copied) (void)hipEventSynchronize(desc_copied);
     if (desc_copied) (void)hipEventDestroy(desc_copied);
     if (desc_host) (void)hipHostFree(desc_host);
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    if (d2h_stream) (void)hipStreamSynchronize(d2h_stream);
+    for (auto &p : pieces) p.release();
     if (stream) (void)hipStreamDestroy(stream);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    if (d2h_stream) (void)hipStreamDestroy(d2h_stream);
     if (params) (void)hipFree(params);
   }
 };
@@ -261,8 +274,21 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
   hipDeviceProp_t props;
   if (hipGetDeviceProperties(&props, dev) != hipSuccess) FAIL(mhcudaRuntimeError);
   gen->num_cus = props.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&gen->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&gen->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&gen->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&gen->d2h_stream, hipStreamNonBlocking) != hipSuccess)
     FAIL(mhcudaRuntimeError);
+  for (auto &p : gen->pieces) {
+    if (p.create() != hipSuccess) FAIL(mhcudaRuntimeError);
+  }
+  if (const char *env = std::getenv("MHCUDA_PIECE_NNZ")) {
+    const unsigned long long v = std::strtoull(env, nullptr, 10);
+    if (v > 0) gen->piece_nnz = v;
+  }
+  if (const char *env = std::getenv("MHCUDA_PIECE_ROWS")) {
+    const unsigned long long v = std::strtoull(env, nullptr, 10);
+    if (v > 0) gen->piece_rows = v;
+  }
   const size_t param_bytes = (size_t)dim * 3u * gen->sp * sizeof(float);
   if (hipMalloc(&gen->params, param_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -400,66 +426,109 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
   if (!cgen || !mats || count == 0) return mhcudaInvalidArguments;
   auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
   int verbosity = gen->verbosity;
-  uint64_t total_nnz = 0, total_rows = 0, total_out = 0;
+  uint64_t total_nnz = 0, total_rows = 0, max_row = 0;
   for (uint32_t m = 0; m < count; m++) {
     const MHCUDAMatrix &x = mats[m];
     if (x.length == 0) continue;
     if (!x.weights || !x.cols || !x.rows || !x.output) return mhcudaInvalidArguments;
     for (uint32_t r = 0; r < x.length; r++) {
       if (x.rows[r + 1] < x.rows[r]) return mhcudaInvalidArguments;  // malformed indptr
+      max_row = std::max<uint64_t>(max_row, x.rows[r + 1] - x.rows[r]);
     }
     total_nnz += x.rows[x.length] - x.rows[0];
-    total_rows += (uint64_t)x.length + 1;
-    total_out += (uint64_t)x.length * gen->samples * 2;
+    total_rows += x.length;
   }
   if (total_rows == 0) return mhcudaInvalidArguments;
   if (gen->samples % 2 != 0) return mhcudaInvalidArguments;
   std::lock_guard<std::mutex> lock(gen->mu);
   HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
   DEBUG("mhcuda_calc: %u matrices, %" PRIu64 " rows, %" PRIu64 " nonzeros\n", count,
-        total_rows - count, total_nnz);
-  // reference fill_buffers (minhashcuda.cc:385-455): grow-only device buffers
-  HIPCH(gen->weights.reserve(std::max<uint64_t>(total_nnz, 1) * 4), mhcudaMemoryAllocationFailure);
-  HIPCH(gen->cols.reserve(std::max<uint64_t>(total_nnz, 1) * 4), mhcudaMemoryAllocationFailure);
-  HIPCH(gen->rows.reserve(total_rows * 4), mhcudaMemoryAllocationFailure);
-  HIPCH(gen->hashes.reserve(std::max<uint64_t>(total_out, 1) * 4), mhcudaMemoryAllocationFailure);
-  std::vector<wmh::MatrixDesc> dev(count);
-  uint64_t nnz_off = 0, row_off = 0, out_off = 0;
-  float *dw = (float *)gen->weights.ptr;
-  uint32_t *dc = (uint32_t *)gen->cols.ptr, *dr = (uint32_t *)gen->rows.ptr,
-           *dh = (uint32_t *)gen->hashes.ptr;
+        total_rows, total_nnz);
+  // Pieces: consecutive row ranges holding at most `budget` nonzeros and
+  // `row_budget` rows (a row is never cut).  Piece p uses device buffer set
+  // p % 2, so the upload of piece p + 1 overlaps the hashing of piece p and
+  // the download of the hashes runs on a third stream.  The grow-only buffers
+  // play the role of the reference's fill_buffers (minhashcuda.cc:385-455).
+  const uint64_t budget = std::max<uint64_t>(gen->piece_nnz, max_row);
+  const uint64_t row_budget = std::max<uint64_t>(1, gen->piece_rows);
+  struct Span {
+    uint32_t m, r0, r1;
+  };
+  std::vector<std::vector<Span>> pieces(1);
+  uint64_t cur_nnz = 0, cur_rows = 0;
   for (uint32_t m = 0; m < count; m++) {
     const MHCUDAMatrix &x = mats[m];
-    dev[m] = wmh::MatrixDesc{nullptr, nullptr, nullptr, nullptr, 0u, 0u, 0u, 0u, 0u, 0u};
-    if (x.length == 0) continue;
-    const uint64_t nnz = x.rows[x.length] - x.rows[0];
-    // the device copy of the matrix starts at CSR position rows[0]
-    if (nnz) {
-      HIPCH(hipMemcpyAsync(dw + nnz_off, x.weights + x.rows[0], nnz * 4,
-                           hipMemcpyHostToDevice, gen->stream), mhcudaMemoryCopyError);
-      HIPCH(hipMemcpyAsync(dc + nnz_off, x.cols + x.rows[0], nnz * 4,
-                           hipMemcpyHostToDevice, gen->stream), mhcudaMemoryCopyError);
+    uint32_t r0 = 0;
+    for (uint32_t r = 0; r < x.length; r++) {
+      const uint64_t n = x.rows[r + 1] - x.rows[r];
+      if (cur_rows > 0 && (cur_nnz + n > budget || cur_rows + 1 > row_budget)) {
+        if (r > r0) pieces.back().push_back(Span{m, r0, r});
+        pieces.emplace_back();
+        r0 = r;
+        cur_nnz = cur_rows = 0;
+      }
+      cur_nnz += n;
+      cur_rows++;
     }
-    HIPCH(hipMemcpyAsync(dr + row_off, x.rows, ((uint64_t)x.length + 1) * 4,
-                         hipMemcpyHostToDevice, gen->stream), mhcudaMemoryCopyError);
-    dev[m].weights = dw + nnz_off;
-    dev[m].cols = dc + nnz_off;
-    dev[m].rows = dr + row_off;
-    dev[m].length = x.length;
-    dev[m].base = x.rows[0];
-    dev[m].output = dh + out_off;
-    nnz_off += nnz;
-    row_off += (uint64_t)x.length + 1;
-    out_off += (uint64_t)x.length * gen->samples * 2;
+    if (x.length > r0) pieces.back().push_back(Span{m, r0, x.length});
   }
-  MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream);
-  if (r != mhcudaSuccess) return r;
-  for (uint32_t m = 0; m < count; m++) {
-    if (mats[m].length == 0) continue;
-    HIPCH(hipMemcpyAsync(mats[m].output, dev[m].output,
-                         (uint64_t)mats[m].length * gen->samples * 2 * 4,
-                         hipMemcpyDeviceToHost, gen->stream), mhcudaMemoryCopyError);
+  const size_t S2 = (size_t)gen->samples * 2;
+  for (size_t p = 0; p < pieces.size(); p++) {
+    const int b = (int)(p & 1);
+    wmh::PieceBuffers &buf = gen->pieces[b];
+    // the set is free once its previous download has completed
+    HIPCH(hipEventSynchronize(buf.done), mhcudaRuntimeError);
+    uint64_t nnz = 0, nrows = 0;
+    for (const Span &sp : pieces[p]) {
+      nnz += mats[sp.m].rows[sp.r1] - mats[sp.m].rows[sp.r0];
+      nrows += sp.r1 - sp.r0;
+    }
+    const size_t nspans = pieces[p].size();
+    HIPCH(buf.weights.reserve(std::max<uint64_t>(nnz, 1) * 4), mhcudaMemoryAllocationFailure);
+    HIPCH(buf.cols.reserve(std::max<uint64_t>(nnz, 1) * 4), mhcudaMemoryAllocationFailure);
+    HIPCH(buf.rows.reserve((nrows + nspans) * 4), mhcudaMemoryAllocationFailure);
+    HIPCH(buf.hashes.reserve(std::max<uint64_t>(nrows * S2, 1) * 4),
+          mhcudaMemoryAllocationFailure);
+    float *dw = (float *)buf.weights.ptr;
+    uint32_t *dc = (uint32_t *)buf.cols.ptr, *dr = (uint32_t *)buf.rows.ptr,
+             *dh = (uint32_t *)buf.hashes.ptr;
+    std::vector<wmh::MatrixDesc> dev(nspans);
+    uint64_t nnz_off = 0, row_off = 0, out_off = 0;
+    for (size_t i = 0; i < nspans; i++) {
+      const Span &sp = pieces[p][i];
+      const MHCUDAMatrix &x = mats[sp.m];
+      const uint32_t first = x.rows[sp.r0];
+      const uint64_t n = x.rows[sp.r1] - first;
+      const uint32_t len = sp.r1 - sp.r0;
+      if (n) {
+        HIPCH(hipMemcpyAsync(dw + nnz_off, x.weights + first, n * 4, hipMemcpyHostToDevice,
+                             gen->copy_stream), mhcudaMemoryCopyError);
+        HIPCH(hipMemcpyAsync(dc + nnz_off, x.cols + first, n * 4, hipMemcpyHostToDevice,
+                             gen->copy_stream), mhcudaMemoryCopyError);
+      }
+      HIPCH(hipMemcpyAsync(dr + row_off, x.rows + sp.r0, ((uint64_t)len + 1) * 4,
+                           hipMemcpyHostToDevice, gen->copy_stream), mhcudaMemoryCopyError);
+      dev[i] = wmh::MatrixDesc{dw + nnz_off, dc + nnz_off, dr + row_off, dh + out_off,
+                               len, first, 0u, 0u, 0u, 0u};
+      nnz_off += n;
+      row_off += (uint64_t)len + 1;
+      out_off += (uint64_t)len * S2;
+    }
+    HIPCH(hipEventRecord(buf.uploaded, gen->copy_stream), mhcudaRuntimeError);
+    HIPCH(hipStreamWaitEvent(gen->stream, buf.uploaded, 0), mhcudaRuntimeError);
+    MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream);
+    if (r != mhcudaSuccess) return r;
+    HIPCH(hipEventRecord(buf.hashed, gen->stream), mhcudaRuntimeError);
+    HIPCH(hipStreamWaitEvent(gen->d2h_stream, buf.hashed, 0), mhcudaRuntimeError);
+    for (size_t i = 0; i < nspans; i++) {
+      const Span &sp = pieces[p][i];
+      HIPCH(hipMemcpyAsync(mats[sp.m].output + (uint64_t)sp.r0 * S2, dev[i].output,
+                           (uint64_t)(sp.r1 - sp.r0) * S2 * 4, hipMemcpyDeviceToHost,
+                           gen->d2h_stream), mhcudaMemoryCopyError);
+    }
+    HIPCH(hipEventRecord(buf.done, gen->d2h_stream), mhcudaRuntimeError);
   }
+  HIPCH(hipStreamSynchronize(gen->d2h_stream), mhcudaRuntimeError);
   HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
   INFO("mhcuda - success\n");
   return mhcudaSuccess;

@@ -19,4 +19,23 @@ struct DevBuf {
   hipError_t reserve(size_t bytes);
 };
 
+// One device buffer set of the pipelined host-memory path, with the events
+// that order its upload, hashing and download.
+struct PieceBuffers {
+  DevBuf weights, cols, rows, hashes;
+  hipEvent_t uploaded = nullptr, hashed = nullptr, done = nullptr;
+  hipError_t create() {
+    hipError_t e = hipEventCreateWithFlags(&uploaded, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&hashed, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    return e;
+  }
+  void release() {
+    if (uploaded) (void)hipEventDestroy(uploaded);
+    if (hashed) (void)hipEventDestroy(hashed);
+    if (done) (void)hipEventDestroy(done);
+    uploaded = hashed = done = nullptr;
+  }
+};
+
 }  // namespace wmh

@@ -118,3 +118,22 @@ def test_host_rejects_malformed_indptr(libmhcuda, capi):
                           3, out.ctypes.data)
     libmhcuda.minhash_cuda_fini(gen)
     assert rc == 1
+
+
+@pytest.mark.parametrize("kernel", [PLAIN, PERSISTENT])
+def test_pipelined_host_path_many_pieces(libmhcuda, batch, expected, monkeypatch, kernel):
+    """Host-memory calls are cut into pieces that alternate between two device
+    buffer sets (upload of one overlapping the hashing of the other).  Tiny
+    piece limits force dozens of pieces, rows larger than the nonzero budget
+    and spans that start mid-matrix; every hash must still equal the oracle's."""
+    params, ref = expected
+    monkeypatch.setenv("MHCUDA_PIECE_NNZ", "20000")
+    monkeypatch.setenv("MHCUDA_PIECE_ROWS", "37")
+    gen = libmhcuda.minhash_cuda_init(1 << 14, 128, deferred=True)
+    libmhcuda.minhash_cuda_assign_vars(gen, *params)
+    got = libmhcuda.minhash_cuda_calc_batch(gen, batch, kernel=kernel)
+    h = libmhcuda.minhash_cuda_calc(gen, batch[2], row_start=11)
+    libmhcuda.minhash_cuda_fini(gen)
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
+    assert np.array_equal(h, ref[2][11:])
