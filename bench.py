@@ -131,6 +131,22 @@ class Engine:
         self.L.minhash_cuda_fini(self.gen)
 
 
+def measured_traffic(workload, kernel, alg_bytes):
+    """HBM bytes per launch of `kernel` on this workload as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
+    profiles/hbm_traffic.json (scaled by a copy-kernel calibration); None when
+    no measurement of this exact configuration is recorded."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["workloads"][workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rec.get("kernel") != kernel or rec.get("algorithmic_bytes") != alg_bytes:
+        return None
+    return rec["traffic_bytes"]
+
+
 def timed_steps(torch, run, engine, steps, warmup):
     """Runs `run` warmup + steps times; returns (wall seconds of the timed
     steps, per-step kernel milliseconds)."""
@@ -230,8 +246,8 @@ def main():
                        **engine.tuning()),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": alg,
-                     "avg_kernel_ms": avg_ms},
+                     "traffic": measured_traffic(args.workload, kname, alg),
+                     "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms},
     })
     if args.workload == "batched" and not args.no_plain:
         ref_out = [o.clone() for o in outs]
