@@ -144,6 +144,50 @@ MHCUDAResult mhcuda_set_timing(MinhashCudaGenerator *gen, int enable);
  * for them to finish); mhcudaInvalidArguments if that call was not timed. */
 MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *gen, float *ms);
 
+/* ------------------------------------------------------------------------ */
+/* Serving loop: continuous batching of CSR requests over one generator.     */
+
+typedef struct MHCUDAServer_ MHCUDAServer;
+
+typedef struct {
+  uint64_t memory_budget;       /* device bytes for requests in flight, split evenly over slots */
+  uint32_t slots;               /* launches in flight at once (one stream each) */
+  uint32_t max_batch_requests;  /* most requests coalesced into one launch */
+} MHCUDAServeConfig;
+
+typedef struct {
+  uint64_t completed;           /* requests hashed since the last reset */
+  uint64_t rejected;            /* requests larger than a slot's memory share */
+  uint64_t batches;             /* launches */
+  double mean_requests_per_batch;
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> hashes in host memory */
+  double window_s;              /* first submit -> last completion */
+  double nnz_samples_per_sec;   /* completed nonzeros * samples / window_s */
+} MHCUDAServeStats;
+
+/* Starts the scheduler and completion threads.  The generator must outlive
+ * the server; other calls may still use it meanwhile. */
+MHCUDAServer *mhcuda_serve_start(MinhashCudaGenerator *gen, const MHCUDAServeConfig *config,
+                                 MHCUDAResult *status);
+
+/* Queues one CSR request (host memory, same conventions as mhcuda_calc);
+ * the arrays must stay valid and `output` untouched until the request is
+ * waited for.  A request that can never fit a slot completes at once with
+ * mhcudaMemoryAllocationFailure. */
+MHCUDAResult mhcuda_serve_submit(MHCUDAServer *server, const float *weights,
+                                 const uint32_t *cols, const uint32_t *rows,
+                                 uint32_t length, uint32_t *output, uint64_t *ticket);
+
+/* Blocks until the request is done; returns its status and (optionally) its
+ * latency.  Each ticket can be waited for once. */
+MHCUDAResult mhcuda_serve_wait(MHCUDAServer *server, uint64_t ticket, double *latency_ms);
+
+/* Latency percentiles and throughput since the last reset (reset != 0 resets). */
+MHCUDAResult mhcuda_serve_stats(MHCUDAServer *server, MHCUDAServeStats *stats, int reset);
+
+/* Hashes everything still queued, then stops the threads and frees the server. */
+MHCUDAResult mhcuda_serve_stop(MHCUDAServer *server);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -32,11 +32,12 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 # host-only translation units: plain C++ against the HIP runtime API
-HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
+HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-pthread"]
 
 SOURCES = [
     ("wmh_kernels.hip", True),
     ("generator.cpp", False),
+    ("serve.cpp", False),
     ("python_module.cpp", False),
 ]
 
@@ -79,7 +80,7 @@ def build(verbose=False):
     pylib = ldlib[3:].split(".so")[0]
     tmp = LIB + ".tmp"
     _run([hipcc(), "-shared", "-fPIC", "-o", tmp] + objs + ["--offload-arch=" + ARCH] +
-         ["-L" + libdir, "-l" + pylib, "-Wl,-rpath," + "/opt/rocm/lib"], verbose)
+         ["-L" + libdir, "-l" + pylib, "-pthread", "-Wl,-rpath," + "/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

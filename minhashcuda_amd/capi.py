@@ -23,6 +23,8 @@ EXPORTS = (
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
     "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_timing", "mhcuda_kernel_time",
+    "mhcuda_serve_start", "mhcuda_serve_submit", "mhcuda_serve_wait", "mhcuda_serve_stats",
+    "mhcuda_serve_stop",
 )
 
 
@@ -37,6 +39,24 @@ class Matrix(ctypes.Structure):
     _fields_ = [("weights", ctypes.c_void_p), ("cols", ctypes.c_void_p),
                 ("rows", ctypes.c_void_p), ("length", ctypes.c_uint32),
                 ("output", ctypes.c_void_p)]
+
+
+class ServeConfig(ctypes.Structure):
+    """MHCUDAServeConfig (minhashcuda.h)."""
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("slots", ctypes.c_uint32),
+                ("max_batch_requests", ctypes.c_uint32)]
+
+
+class ServeStats(ctypes.Structure):
+    """MHCUDAServeStats (minhashcuda.h)."""
+    _fields_ = [("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("mean_requests_per_batch", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("window_s", ctypes.c_double), ("nnz_samples_per_sec", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def load(path):
@@ -69,6 +89,16 @@ def load(path):
     L.mhcuda_set_timing.argtypes = [vp, i32]
     L.mhcuda_kernel_time.restype = i32
     L.mhcuda_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.mhcuda_serve_start.restype = vp
+    L.mhcuda_serve_start.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(i32)]
+    L.mhcuda_serve_submit.restype = i32
+    L.mhcuda_serve_submit.argtypes = [vp, vp, vp, vp, u32, vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.mhcuda_serve_wait.restype = i32
+    L.mhcuda_serve_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    L.mhcuda_serve_stats.restype = i32
+    L.mhcuda_serve_stats.argtypes = [vp, ctypes.POINTER(ServeStats), i32]
+    L.mhcuda_serve_stop.restype = i32
+    L.mhcuda_serve_stop.argtypes = [vp]
     return L
 
 

@@ -67,53 +67,19 @@ hipError_t DevBuf::reserve(size_t bytes) {
 
 using wmh::DevBuf;
 
-struct MinhashCudaGenerator_ {
-  uint32_t dim = 0;
-  uint16_t samples = 0;
-  int verbosity = 0;
-  int device = -1;
-  uint32_t sp = 0;
-  int num_cus = 0;
-  float *params = nullptr;            // [dim][3][sp]
-  hipStream_t stream = nullptr;
-  std::mutex mu;
-  // grow-only caches, reference minhashcuda.cc:27-31
-  DevBuf staging, plan;
-  // host-memory calls: two device buffer sets used alternately, their own
-  // upload and download streams, and the size limits of one piece
-  wmh::PieceBuffers pieces[2];
-  hipStream_t copy_stream = nullptr, d2h_stream = nullptr;
-  uint64_t piece_nnz = 1ull << 28;
-  uint64_t piece_rows = 1ull << 20;
-  wmh::Tuning tuning{2, 2048, 16};
-  int kernel = mhcudaKernelAuto;
-  // pinned staging of the batch descriptors; `desc_copied` marks the end of
-  // the last copy out of it so that it is not overwritten too early
-  wmh::MatrixDesc *desc_host = nullptr;
-  size_t desc_host_cap = 0;
-  hipEvent_t desc_copied = nullptr;
-  // optional timing of the hashing kernel(s) of the last call
-  bool timing = false;
-  bool timed = false;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-
-  ~MinhashCudaGenerator_() {
-    if (ev_begin) (void)hipEventDestroy(ev_begin);
-    if (ev_end) (void)hipEventDestroy(ev_end);
-    if (device >= 0) (void)hipSetDevice(device);
-    if (desc_copied) (void)hipEventSynchronize(desc_copied);
-    if (desc_copied) (void)hipEventDestroy(desc_copied);
-    if (desc_host) (void)hipHostFree(desc_host);
-    if (stream) (void)hipStreamSynchronize(stream);
-    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
-    if (d2h_stream) (void)hipStreamSynchronize(d2h_stream);
-    for (auto &p : pieces) p.release();
-    if (stream) (void)hipStreamDestroy(stream);
-    if (copy_stream) (void)hipStreamDestroy(copy_stream);
-    if (d2h_stream) (void)hipStreamDestroy(d2h_stream);
-    if (params) (void)hipFree(params);
-  }
-};
+MinhashCudaGenerator_::~MinhashCudaGenerator_() {
+  if (ev_begin) (void)hipEventDestroy(ev_begin);
+  if (ev_end) (void)hipEventDestroy(ev_end);
+  if (device >= 0) (void)hipSetDevice(device);
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+  if (d2h_stream) (void)hipStreamSynchronize(d2h_stream);
+  for (auto &p : pieces) p.release();
+  if (stream) (void)hipStreamDestroy(stream);
+  if (copy_stream) (void)hipStreamDestroy(copy_stream);
+  if (d2h_stream) (void)hipStreamDestroy(d2h_stream);
+  if (params) (void)hipFree(params);
+}
 
 namespace {
 
@@ -178,7 +144,7 @@ namespace wmh {
 // Enqueues the persistent batched path on `s`: descriptors go through the
 // pinned staging buffer into device memory next to the control block.
 static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                                    hipStream_t s) {
+                                    hipStream_t s, PlanCtx *ctx) {
   int verbosity = gen->verbosity;
   const uint32_t count = (uint32_t)mats.size();
   uint64_t chunks = 0;
@@ -190,23 +156,23 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   if (chunks >= (1ull << 32)) return mhcudaInvalidArguments;
   const size_t desc_bytes = sizeof(MatrixDesc) * count;
   const size_t ctl_off = (desc_bytes + 255) & ~(size_t)255;
-  HIPCH(gen->plan.reserve(ctl_off + sizeof(BatchCtl)), mhcudaMemoryAllocationFailure);
-  if (!gen->desc_copied)
-    HIPCH(hipEventCreateWithFlags(&gen->desc_copied, hipEventDisableTiming), mhcudaRuntimeError);
-  HIPCH(hipEventSynchronize(gen->desc_copied), mhcudaRuntimeError);
-  if (gen->desc_host_cap < count) {
-    if (gen->desc_host) (void)hipHostFree(gen->desc_host);
-    gen->desc_host = nullptr;
-    gen->desc_host_cap = 0;
-    HIPCH(hipHostMalloc((void **)&gen->desc_host, desc_bytes, hipHostMallocDefault),
+  if (!ctx->copied)
+    HIPCH(hipEventCreateWithFlags(&ctx->copied, hipEventDisableTiming), mhcudaRuntimeError);
+  HIPCH(hipEventSynchronize(ctx->copied), mhcudaRuntimeError);
+  HIPCH(ctx->plan.reserve(ctl_off + sizeof(BatchCtl)), mhcudaMemoryAllocationFailure);
+  if (ctx->cap < count) {
+    if (ctx->desc_host) (void)hipHostFree(ctx->desc_host);
+    ctx->desc_host = nullptr;
+    ctx->cap = 0;
+    HIPCH(hipHostMalloc((void **)&ctx->desc_host, desc_bytes, hipHostMallocDefault),
           mhcudaMemoryAllocationFailure);
-    gen->desc_host_cap = count;
+    ctx->cap = count;
   }
-  std::memcpy(gen->desc_host, mats.data(), desc_bytes);
-  char *dev = (char *)gen->plan.ptr;
-  HIPCH(hipMemcpyAsync(dev, gen->desc_host, desc_bytes, hipMemcpyHostToDevice, s),
+  std::memcpy(ctx->desc_host, mats.data(), desc_bytes);
+  char *dev = (char *)ctx->plan.ptr;
+  HIPCH(hipMemcpyAsync(dev, ctx->desc_host, desc_bytes, hipMemcpyHostToDevice, s),
         mhcudaMemoryCopyError);
-  HIPCH(hipEventRecord(gen->desc_copied, s), mhcudaRuntimeError);
+  HIPCH(hipEventRecord(ctx->copied, s), mhcudaRuntimeError);
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
                           (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), gen->tuning,
                           gen->num_cus, s, gen->timing ? gen->ev_begin : nullptr,
@@ -217,8 +183,16 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
 }
 
 // Launches the chosen hashing kernel over device-resident matrices on `s`.
-MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen,
-                               std::vector<MatrixDesc> &mats, int kernel, hipStream_t s) {
+PlanCtx::~PlanCtx() {
+  if (copied) {
+    (void)hipEventSynchronize(copied);
+    (void)hipEventDestroy(copied);
+  }
+  if (desc_host) (void)hipHostFree(desc_host);
+}
+
+MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
+                               int kernel, hipStream_t s, PlanCtx *ctx) {
   int verbosity = gen->verbosity;
   if (gen->samples % 2 != 0) {
     // the reference finds no even sample_delta for odd counts
@@ -227,7 +201,7 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen,
   }
   if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
   if (kernel == mhcudaKernelAuto) kernel = mhcudaKernelPersistent;
-  if (kernel == mhcudaKernelPersistent) return hash_persistent(gen, mats, s);
+  if (kernel == mhcudaKernelPersistent) return hash_persistent(gen, mats, s, ctx);
   if (kernel != mhcudaKernelPlain) return mhcudaInvalidArguments;
   if (gen->timing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
   for (const MatrixDesc &m : mats) {
@@ -415,7 +389,7 @@ MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t
     descs[m] = wmh::MatrixDesc{mats[m].weights, mats[m].cols, mats[m].rows, mats[m].output,
                                mats[m].length, 0u, 0u, 0u, 0u, 0u};
   }
-  MHCUDAResult r = wmh::hash_device_batch(gen, descs, kernel, s);
+  MHCUDAResult r = wmh::hash_device_batch(gen, descs, kernel, s, &gen->plan);
   if (r != mhcudaSuccess) return r;
   if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
   return mhcudaSuccess;
@@ -516,7 +490,7 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     }
     HIPCH(hipEventRecord(buf.uploaded, gen->copy_stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->stream, buf.uploaded, 0), mhcudaRuntimeError);
-    MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream);
+    MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream, &gen->plan);
     if (r != mhcudaSuccess) return r;
     HIPCH(hipEventRecord(buf.hashed, gen->stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->d2h_stream, buf.hashed, 0), mhcudaRuntimeError);

@@ -1,8 +1,16 @@
-// Host-side helpers shared by the runtime sources (not part of the C ABI).
+// Host-side runtime internals shared by the runtime sources (not part of the
+// C ABI): device buffers, descriptor staging and the generator object.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "minhashcuda.h"
+#include "wmh_kernels.h"
 
 namespace wmh {
 
@@ -37,5 +45,61 @@ struct PieceBuffers {
     uploaded = hashed = done = nullptr;
   }
 };
+
+// Descriptor staging of one stream's persistent launches: pinned host copy,
+// device copy plus control block, and the event marking the end of the last
+// upload out of the pinned copy (so that it is not overwritten too early).
+struct PlanCtx {
+  DevBuf plan;
+  MatrixDesc *desc_host = nullptr;
+  size_t cap = 0;
+  hipEvent_t copied = nullptr;
+  PlanCtx() = default;
+  PlanCtx(const PlanCtx &) = delete;
+  PlanCtx &operator=(const PlanCtx &) = delete;
+  ~PlanCtx();
+};
+
+}  // namespace wmh
+
+struct MinhashCudaGenerator_ {
+  uint32_t dim = 0;
+  uint16_t samples = 0;
+  int verbosity = 0;
+  int device = -1;
+  uint32_t sp = 0;
+  int num_cus = 0;
+  float *params = nullptr;            // [dim][3][sp]
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  // grow-only caches, reference minhashcuda.cc:27-31
+  wmh::DevBuf staging;
+  wmh::PlanCtx plan;  // descriptors of the calls made on `stream`
+  // host-memory calls: two device buffer sets used alternately, their own
+  // upload and download streams, and the size limits of one piece
+  wmh::PieceBuffers pieces[2];
+  hipStream_t copy_stream = nullptr, d2h_stream = nullptr;
+  uint64_t piece_nnz = 1ull << 28;
+  uint64_t piece_rows = 1ull << 20;
+  wmh::Tuning tuning{2, 2048, 16};
+  int kernel = mhcudaKernelAuto;
+  // optional timing of the hashing kernel(s) of the last call
+  bool timing = false;
+  bool timed = false;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+
+  MinhashCudaGenerator_() = default;
+  MinhashCudaGenerator_(const MinhashCudaGenerator_ &) = delete;
+  MinhashCudaGenerator_ &operator=(const MinhashCudaGenerator_ &) = delete;
+  ~MinhashCudaGenerator_();
+};
+
+namespace wmh {
+
+// Enqueues the chosen hashing kernel over device-resident matrices on `s`,
+// staging persistent-launch descriptors through `ctx`.  The caller owns the
+// synchronization of `ctx` with other streams.
+MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
+                               int kernel, hipStream_t s, PlanCtx *ctx);
 
 }  // namespace wmh

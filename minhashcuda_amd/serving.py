@@ -1,0 +1,71 @@
+"""Python client of the engine's serving loop (mhcuda_serve_* in minhashcuda.h).
+
+    server = Server(gen, memory_budget=1 << 30, slots=3)
+    ticket = server.submit(csr)          # scipy csr_matrix, float32 data
+    hashes = server.wait(ticket)         # (rows, samples, 2) uint32
+    print(server.stats())
+    server.stop()
+
+Requests are hashed by continuous batching: whatever is pending when a slot
+frees up goes into one persistent launch; slots overlap on the device.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from . import capi as capi_mod
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libMHCUDA.so")
+
+
+class Server:
+    def __init__(self, gen, memory_budget=1 << 30, slots=3, max_batch_requests=256, lib=None):
+        self.lib = lib or capi_mod.load(LIB_PATH)
+        cfg = capi_mod.ServeConfig(memory_budget, slots, max_batch_requests)
+        status = ctypes.c_int(0)
+        self.handle = self.lib.mhcuda_serve_start(ctypes.c_void_p(gen), ctypes.byref(cfg),
+                                                  ctypes.byref(status))
+        if not self.handle:
+            raise RuntimeError("mhcuda_serve_start failed: %s"
+                               % capi_mod.RESULT_NAMES.get(status.value, status.value))
+        params = self.lib.mhcuda_get_parameters(ctypes.c_void_p(gen))
+        self.samples = params.samples
+        self._keep = {}  # ticket -> arrays that must outlive the request
+        self._lock = threading.Lock()
+
+    def submit(self, m):
+        """Queues a scipy CSR matrix; returns its ticket."""
+        w = np.ascontiguousarray(m.data, dtype=np.float32)
+        c = np.ascontiguousarray(m.indices, dtype=np.uint32)
+        r = np.ascontiguousarray(m.indptr, dtype=np.uint32)
+        out = np.empty((m.shape[0], self.samples, 2), np.uint32)
+        t = ctypes.c_uint64(0)
+        capi_mod.check(self.lib.mhcuda_serve_submit(
+            self.handle, w.ctypes.data, c.ctypes.data, r.ctypes.data, m.shape[0],
+            out.ctypes.data, ctypes.byref(t)), "mhcuda_serve_submit")
+        with self._lock:
+            self._keep[t.value] = (w, c, r, out)
+        return t.value
+
+    def wait(self, ticket, with_latency=False):
+        """Blocks until the request is hashed; returns its hashes (and latency ms)."""
+        lat = ctypes.c_double(0)
+        rc = self.lib.mhcuda_serve_wait(self.handle, ticket, ctypes.byref(lat))
+        with self._lock:
+            keep = self._keep.pop(ticket, None)
+        capi_mod.check(rc, "request %d" % ticket)
+        return (keep[3], lat.value) if with_latency else keep[3]
+
+    def stats(self, reset=False):
+        s = capi_mod.ServeStats()
+        capi_mod.check(self.lib.mhcuda_serve_stats(self.handle, ctypes.byref(s), int(reset)),
+                       "mhcuda_serve_stats")
+        return s.as_dict()
+
+    def stop(self):
+        if self.handle:
+            self.lib.mhcuda_serve_stop(self.handle)
+            self.handle = None
+            self._keep.clear()
