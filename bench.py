@@ -47,6 +47,8 @@ def parse():
                    help="serve: comma-separated arrival rates (requests/s)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
                    help="serve: duration of each arrival-rate run")
+    p.add_argument("--pinned", action="store_true",
+                   help="serve: requests live in pinned host memory (direct DMA)")
     p.add_argument("--spl", type=int, default=0, help="samples per lane (0: default)")
     p.add_argument("--seg", type=int, default=0, help="nonzeros per work item (0: default)")
     p.add_argument("--wpc", type=int, default=0, help="persistent waves per CU (0: default)")
@@ -191,6 +193,20 @@ def serve_bench(args, torch):
     dim, samples = 1 << 22, 256
     log("serve: generating request pool")
     pool = serve_requests(256, dim, seed=5)
+    pinned_pool = []
+    if args.pinned:
+        for m in pool:
+            arrs = []
+            for a in (m.data.astype(np.float32), m.indices.astype(np.uint32),
+                      m.indptr.astype(np.uint32)):
+                t = torch.empty(a.shape, dtype=torch.float32 if a.dtype == np.float32
+                                else torch.int32, pin_memory=True)
+                v = t.numpy().view(a.dtype)
+                v[...] = a
+                arrs.append((t, v))
+            ot = torch.empty((m.shape[0], samples, 2), dtype=torch.int32, pin_memory=True)
+            arrs.append((ot, ot.numpy().view(np.uint32)))
+            pinned_pool.append(arrs)
     gen = libMHCUDA.minhash_cuda_init(dim, samples, seed=1, devices=1)
     server = serving.Server(gen, memory_budget=4 << 30, slots=3, max_batch_requests=256)
     # warm-up: every buffer reaches its working size
@@ -226,7 +242,11 @@ def serve_bench(args, torch):
                     break
                 if due - now > 0.0005:
                     time.sleep(due - now - 0.0003)
-            tickets.append(server.submit(pool[i % len(pool)]))
+            if args.pinned:
+                a = pinned_pool[i % len(pool)]
+                tickets.append(server.submit_arrays(a[0][1], a[1][1], a[2][1], a[3][1]))
+            else:
+                tickets.append(server.submit(pool[i % len(pool)]))
         submit_span = time.perf_counter() - t0
         th.join()
         st = server.stats(reset=True)
@@ -252,7 +272,9 @@ def serve_bench(args, torch):
         "config": {"workload": "serving: Poisson arrivals, requests of 1-4096 rows "
                                "(log-uniform), dim 2^22, nnz/row lognormal(median 200, sigma 1) "
                                "clipped to 50000, lognormal(0,1) weights, 256 samples, "
-                               "parameters resident (12.9 GB), 3 slots, 4 GiB budget",
+                               "parameters resident (12.9 GB), 3 slots, 4 GiB budget, "
+                               "request arrays in %s host memory"
+                               % ("pinned" if args.pinned else "pageable"),
                    "pool_requests": len(pool)},
         "sweep": results}), flush=True)
 

@@ -129,7 +129,10 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *gen, uint32_t count,
                                const MHCUDAMatrix *mats, int kernel);
 
 /* Tuning knobs of the generator (samples per lane 1/2/4, nonzeros per
- * persistent work item, persistent waves per CU); 0 keeps the current value. */
+ * persistent work item, persistent waves per CU); 0 keeps the current value.
+ * An explicit segment fixes the item size; segment = -1 restores the default
+ * behaviour, where calls whose nonzero count the host knows (host-memory
+ * calls, serving) shrink it for small batches. */
 MHCUDAResult mhcuda_set_tuning(MinhashCudaGenerator *gen, int samples_per_lane,
                                int segment, int waves_per_cu);
 MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen,

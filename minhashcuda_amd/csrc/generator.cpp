@@ -143,8 +143,20 @@ namespace wmh {
 
 // Enqueues the persistent batched path on `s`: descriptors go through the
 // pinned staging buffer into device memory next to the control block.
+// Item size for a batch of `nnz` nonzeros: the configured size, shrunk (in
+// steps of 64, down to 256) until the batch yields about four work items per
+// resident wavefront, so that small batches still spread over every CU.
+uint32_t item_size(const MinhashCudaGenerator *gen, uint64_t nnz) {
+  uint32_t seg = (uint32_t)gen->tuning.seg;
+  if (!gen->seg_auto || nnz == 0) return seg;
+  const uint64_t waves = (uint64_t)std::max(1, gen->num_cus) * gen->tuning.waves_per_cu;
+  const uint64_t want = nnz / (waves * 4);
+  if (want < seg) seg = (uint32_t)std::max<uint64_t>(256, want & ~(uint64_t)63);
+  return std::min<uint32_t>(seg, (uint32_t)gen->tuning.seg);
+}
+
 static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                                    hipStream_t s, PlanCtx *ctx) {
+                                    hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint) {
   int verbosity = gen->verbosity;
   const uint32_t count = (uint32_t)mats.size();
   uint64_t chunks = 0;
@@ -173,8 +185,10 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   HIPCH(hipMemcpyAsync(dev, ctx->desc_host, desc_bytes, hipMemcpyHostToDevice, s),
         mhcudaMemoryCopyError);
   HIPCH(hipEventRecord(ctx->copied, s), mhcudaRuntimeError);
+  Tuning tuning = gen->tuning;
+  tuning.seg = (int)item_size(gen, nnz_hint);
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
-                          (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), gen->tuning,
+                          (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), tuning,
                           gen->num_cus, s, gen->timing ? gen->ev_begin : nullptr,
                           gen->timing ? gen->ev_end : nullptr),
         mhcudaRuntimeError);
@@ -192,7 +206,7 @@ PlanCtx::~PlanCtx() {
 }
 
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx) {
+                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint) {
   int verbosity = gen->verbosity;
   if (gen->samples % 2 != 0) {
     // the reference finds no even sample_delta for odd counts
@@ -201,7 +215,7 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
   }
   if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
   if (kernel == mhcudaKernelAuto) kernel = mhcudaKernelPersistent;
-  if (kernel == mhcudaKernelPersistent) return hash_persistent(gen, mats, s, ctx);
+  if (kernel == mhcudaKernelPersistent) return hash_persistent(gen, mats, s, ctx, nnz_hint);
   if (kernel != mhcudaKernelPlain) return mhcudaInvalidArguments;
   if (gen->timing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
   for (const MatrixDesc &m : mats) {
@@ -389,7 +403,7 @@ MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t
     descs[m] = wmh::MatrixDesc{mats[m].weights, mats[m].cols, mats[m].rows, mats[m].output,
                                mats[m].length, 0u, 0u, 0u, 0u, 0u};
   }
-  MHCUDAResult r = wmh::hash_device_batch(gen, descs, kernel, s, &gen->plan);
+  MHCUDAResult r = wmh::hash_device_batch(gen, descs, kernel, s, &gen->plan, 0);
   if (r != mhcudaSuccess) return r;
   if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
   return mhcudaSuccess;
@@ -490,7 +504,7 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     }
     HIPCH(hipEventRecord(buf.uploaded, gen->copy_stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->stream, buf.uploaded, 0), mhcudaRuntimeError);
-    MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream, &gen->plan);
+    MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream, &gen->plan, nnz);
     if (r != mhcudaSuccess) return r;
     HIPCH(hipEventRecord(buf.hashed, gen->stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->d2h_stream, buf.hashed, 0), mhcudaRuntimeError);
@@ -513,11 +527,17 @@ MHCUDAResult mhcuda_set_tuning(MinhashCudaGenerator *gen, int samples_per_lane,
   if (!gen) return mhcudaInvalidArguments;
   if (samples_per_lane && samples_per_lane != 1 && samples_per_lane != 2 && samples_per_lane != 4)
     return mhcudaInvalidArguments;
-  if (segment < 0 || (segment % 64) != 0 || waves_per_cu < 0 || waves_per_cu > 32)
+  if (segment < -1 || (segment > 0 && segment % 64 != 0) || waves_per_cu < 0 ||
+      waves_per_cu > 32)
     return mhcudaInvalidArguments;
   std::lock_guard<std::mutex> lock(gen->mu);
   if (samples_per_lane) gen->tuning.spl = samples_per_lane;
-  if (segment) gen->tuning.seg = segment;
+  if (segment > 0) {
+    gen->tuning.seg = segment;
+    gen->seg_auto = false;
+  } else if (segment == -1) {
+    gen->seg_auto = true;
+  }
   if (waves_per_cu) gen->tuning.waves_per_cu = waves_per_cu;
   return mhcudaSuccess;
 }

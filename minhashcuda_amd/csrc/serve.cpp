@@ -5,11 +5,12 @@
 // scheduler thread admits pending requests (FIFO) into a free slot up to the
 // slot's share of the device memory budget, uploads them, and enqueues one
 // persistent hashing launch for the whole group plus the download of every
-// output on the slot's stream.  Slots run concurrently on the device, so a
-// request that arrives while others are being hashed is admitted as soon as a
-// slot frees up and its work overlaps theirs.  A completion thread retires
-// slots in launch order and records each request's latency (submit -> hashes
-// in host memory).  Each hash equals what a lone mhcuda_calc of that request
+// output on the slot's stream.  Each slot has a worker thread that stages
+// the uploads (pinned arrays go straight to DMA), waits for its launch and
+// copies the hashes out, so slots overlap both on the device and on the
+// host; a request that arrives while others are being hashed is admitted as
+// soon as a slot frees up.  Each request's latency is submit -> hashes in
+// host memory.  Each hash equals what a lone mhcuda_calc of that request
 // gives: rows are hashed independently of whatever shares the launch.
 #include <hip/hip_runtime.h>
 
@@ -47,19 +48,49 @@ struct Request {
   double latency_ms = 0;
 };
 
+// Grow-only pinned host buffer.
+struct PinnedBuf {
+  char *ptr = nullptr;
+  size_t cap = 0;
+  bool reserve(size_t bytes) {
+    if (bytes <= cap) return true;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    if (hipHostMalloc((void **)&ptr, bytes, hipHostMallocDefault) != hipSuccess) return false;
+    cap = bytes;
+    return true;
+  }
+  ~PinnedBuf() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+};
+
+bool is_pinned(const void *p) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is unknown to the runtime
+    return false;
+  }
+  return attr.type == hipMemoryTypeHost;
+}
+
+// One launch in flight: its own stream, device buffer, descriptor staging,
+// pinned landing zones and worker thread.  The worker uploads the admitted
+// requests, enqueues their hashing and downloads, waits for them and hands
+// the hashes back, so host-side copies of different slots run in parallel
+// while their launches overlap on the device.
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t finished = nullptr;
-  wmh::DevBuf buffer;   // inputs and outputs of the requests in flight
+  wmh::DevBuf buffer;
   wmh::PlanCtx plan;
-  // pinned landing zone of the hashes: a download into pageable memory would
-  // block the scheduler until the launch finished
-  char *host_out = nullptr;
-  size_t host_out_cap = 0;
-  std::vector<size_t> out_offsets;
+  PinnedBuf host_in;    // staging of pageable inputs
+  PinnedBuf host_out;   // landing zone of hashes bound for pageable outputs
   std::vector<Request *> reqs;
-  MHCUDAResult status = mhcudaSuccess;
-  bool busy = false;
+  bool busy = false;      // requests assigned and not yet retired
+  std::thread worker;
+  std::condition_variable cv;
 };
 
 uint64_t align256(uint64_t x) { return (x + 255u) & ~(uint64_t)255u; }
@@ -73,15 +104,13 @@ struct MHCUDAServer_ {
   std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
   std::condition_variable cv_work;   // scheduler: pending request or free slot
-  std::condition_variable cv_inflight;  // completer: a slot was launched
   std::condition_variable cv_done;   // waiters: a request finished
   std::deque<Request *> pending;
-  std::deque<Slot *> inflight;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
   uint64_t next_ticket = 1;
   bool stopping = false;
   bool scheduler_done = false;
-  std::thread scheduler, completer;
+  std::thread scheduler;
   // statistics since the last reset
   std::vector<double> latencies;
   uint64_t rejected = 0, batches = 0, batched_requests = 0, units = 0;
@@ -93,14 +122,42 @@ struct MHCUDAServer_ {
            align256((uint64_t)length * gen->samples * 8);
   }
 
-  // Uploads the requests of `slot`, enqueues their hashing and downloads.
-  MHCUDAResult launch(Slot *slot) {
-    const int verbosity = gen->verbosity;
+  // Uploads, hashes and downloads the requests of `slot`; returns when the
+  // hashes are in the requests' output arrays.
+  MHCUDAResult run(Slot *slot) {
     char *base = (char *)slot->buffer.ptr;
-    uint64_t off = 0;
+    const uint64_t S8 = (uint64_t)gen->samples * 8;
+    // host staging sizes for pageable inputs and outputs
+    size_t in_bytes = 0, out_bytes = 0;
+    std::vector<char> in_pinned(slot->reqs.size() * 3), out_pinned(slot->reqs.size());
+    for (size_t i = 0; i < slot->reqs.size(); i++) {
+      Request *r = slot->reqs[i];
+      const uint32_t first = r->rows[0];
+      in_pinned[3 * i] = r->nnz == 0 || is_pinned(r->weights + first);
+      in_pinned[3 * i + 1] = r->nnz == 0 || is_pinned(r->cols + first);
+      in_pinned[3 * i + 2] = is_pinned(r->rows);
+      if (!in_pinned[3 * i]) in_bytes += align256(r->nnz * 4);
+      if (!in_pinned[3 * i + 1]) in_bytes += align256(r->nnz * 4);
+      if (!in_pinned[3 * i + 2]) in_bytes += align256(((uint64_t)r->length + 1) * 4);
+      out_pinned[i] = is_pinned(r->output);
+      if (!out_pinned[i]) out_bytes += align256(r->length * S8);
+    }
+    if (!slot->host_in.reserve(in_bytes) || !slot->host_out.reserve(out_bytes))
+      return mhcudaMemoryAllocationFailure;
+    auto upload = [&](void *dst, const void *src, size_t n, bool pinned, size_t &staged) {
+      if (!pinned) {
+        std::memcpy(slot->host_in.ptr + staged, src, n);
+        src = slot->host_in.ptr + staged;
+        staged += align256(n);
+      }
+      return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, slot->stream) == hipSuccess;
+    };
+    uint64_t off = 0, total_nnz = 0;
+    size_t staged = 0;
     std::vector<wmh::MatrixDesc> descs;
     descs.reserve(slot->reqs.size());
-    for (Request *r : slot->reqs) {
+    for (size_t i = 0; i < slot->reqs.size(); i++) {
+      Request *r = slot->reqs[i];
       const uint32_t first = r->rows[0];
       float *dw = (float *)(base + off);
       off += align256(r->nnz * 4);
@@ -109,52 +166,72 @@ struct MHCUDAServer_ {
       uint32_t *dr = (uint32_t *)(base + off);
       off += align256(((uint64_t)r->length + 1) * 4);
       uint32_t *dh = (uint32_t *)(base + off);
-      off += align256((uint64_t)r->length * gen->samples * 8);
-      if (r->nnz) {
-        if (hipMemcpyAsync(dw, r->weights + first, r->nnz * 4, hipMemcpyHostToDevice,
-                           slot->stream) != hipSuccess ||
-            hipMemcpyAsync(dc, r->cols + first, r->nnz * 4, hipMemcpyHostToDevice,
-                           slot->stream) != hipSuccess)
-          return mhcudaMemoryCopyError;
-      }
-      if (hipMemcpyAsync(dr, r->rows, ((uint64_t)r->length + 1) * 4, hipMemcpyHostToDevice,
-                         slot->stream) != hipSuccess)
+      off += align256(r->length * S8);
+      if (r->nnz && (!upload(dw, r->weights + first, r->nnz * 4, in_pinned[3 * i], staged) ||
+                     !upload(dc, r->cols + first, r->nnz * 4, in_pinned[3 * i + 1], staged)))
+        return mhcudaMemoryCopyError;
+      if (!upload(dr, r->rows, ((uint64_t)r->length + 1) * 4, in_pinned[3 * i + 2], staged))
         return mhcudaMemoryCopyError;
       descs.push_back(wmh::MatrixDesc{dw, dc, dr, dh, r->length, first, 0u, 0u, 0u, 0u});
+      total_nnz += r->nnz;
     }
-    MHCUDAResult res =
-        wmh::hash_device_batch(gen, descs, mhcudaKernelPersistent, slot->stream, &slot->plan);
+    MHCUDAResult res = wmh::hash_device_batch(gen, descs, mhcudaKernelPersistent, slot->stream,
+                                              &slot->plan, total_nnz);
     if (res != mhcudaSuccess) return res;
-    size_t out_bytes = 0;
-    slot->out_offsets.clear();
-    for (Request *r : slot->reqs) {
-      slot->out_offsets.push_back(out_bytes);
-      out_bytes += (uint64_t)r->length * gen->samples * 8;
-    }
-    if (out_bytes > slot->host_out_cap) {
-      if (slot->host_out) (void)hipHostFree(slot->host_out);
-      slot->host_out = nullptr;
-      slot->host_out_cap = 0;
-      if (hipHostMalloc((void **)&slot->host_out, out_bytes, hipHostMallocDefault) != hipSuccess)
-        return mhcudaMemoryAllocationFailure;
-      slot->host_out_cap = out_bytes;
-    }
-    for (size_t i = 0; i < descs.size(); i++) {
+    std::vector<size_t> out_off(slot->reqs.size());
+    size_t landed = 0;
+    for (size_t i = 0; i < slot->reqs.size(); i++) {
       Request *r = slot->reqs[i];
-      if (hipMemcpyAsync(slot->host_out + slot->out_offsets[i], descs[i].output,
-                         (uint64_t)r->length * gen->samples * 8, hipMemcpyDeviceToHost,
+      void *dst = r->output;
+      if (!out_pinned[i]) {
+        out_off[i] = landed;
+        dst = slot->host_out.ptr + landed;
+        landed += align256(r->length * S8);
+      }
+      if (hipMemcpyAsync(dst, descs[i].output, r->length * S8, hipMemcpyDeviceToHost,
                          slot->stream) != hipSuccess)
         return mhcudaMemoryCopyError;
     }
-    if (hipEventRecord(slot->finished, slot->stream) != hipSuccess) {
-      if (verbosity > 0) printf("serve: failed to record a slot event\n");
+    if (hipEventRecord(slot->finished, slot->stream) != hipSuccess ||
+        hipEventSynchronize(slot->finished) != hipSuccess)
       return mhcudaRuntimeError;
+    for (size_t i = 0; i < slot->reqs.size(); i++) {
+      if (!out_pinned[i])
+        std::memcpy(slot->reqs[i]->output, slot->host_out.ptr + out_off[i],
+                    slot->reqs[i]->length * S8);
     }
     return mhcudaSuccess;
   }
 
-  void schedule_loop() {
+  void worker_loop(Slot *slot) {
     (void)hipSetDevice(gen->device);
+    std::unique_lock<std::mutex> lock(mu);
+    for (;;) {
+      slot->cv.wait(lock, [&] { return !slot->reqs.empty() || scheduler_done; });
+      if (slot->reqs.empty()) break;  // scheduler finished and nothing assigned
+      lock.unlock();
+      MHCUDAResult st = run(slot);
+      if (st != mhcudaSuccess) (void)hipStreamSynchronize(slot->stream);
+      const Clock::time_point now = Clock::now();
+      lock.lock();
+      for (Request *r : slot->reqs) {
+        r->status = st;
+        r->done = true;
+        r->latency_ms = std::chrono::duration<double, std::milli>(now - r->submitted).count();
+        if (st == mhcudaSuccess) {
+          latencies.push_back(r->latency_ms);
+          units += r->nnz * gen->samples;
+        }
+      }
+      if (now > last_done) last_done = now;
+      slot->reqs.clear();
+      slot->busy = false;
+      cv_work.notify_one();
+      cv_done.notify_all();
+    }
+  }
+
+  void schedule_loop() {
     std::unique_lock<std::mutex> lock(mu);
     for (;;) {
       Slot *slot = nullptr;
@@ -174,7 +251,6 @@ struct MHCUDAServer_ {
       }
       // admission: FIFO while the slot's memory budget and request cap allow
       uint64_t used = 0;
-      slot->reqs.clear();
       while (!pending.empty() && slot->reqs.size() < cfg.max_batch_requests &&
              used + pending.front()->bytes <= slot_bytes) {
         used += pending.front()->bytes;
@@ -184,53 +260,10 @@ struct MHCUDAServer_ {
       slot->busy = true;
       batches++;
       batched_requests += slot->reqs.size();
-      lock.unlock();
-      slot->status = launch(slot);
-      if (slot->status != mhcudaSuccess) (void)hipStreamSynchronize(slot->stream);
-      lock.lock();
-      inflight.push_back(slot);
-      cv_inflight.notify_one();
+      slot->cv.notify_one();
     }
     scheduler_done = true;
-    cv_inflight.notify_one();
-  }
-
-  void complete_loop() {
-    (void)hipSetDevice(gen->device);
-    std::unique_lock<std::mutex> lock(mu);
-    for (;;) {
-      cv_inflight.wait(lock, [&] { return !inflight.empty() || scheduler_done; });
-      if (inflight.empty()) break;  // scheduler finished and nothing in flight
-      Slot *slot = inflight.front();
-      lock.unlock();
-      MHCUDAResult st = slot->status;
-      if (st == mhcudaSuccess && hipEventSynchronize(slot->finished) != hipSuccess)
-        st = mhcudaRuntimeError;
-      if (st == mhcudaSuccess) {
-        for (size_t i = 0; i < slot->reqs.size(); i++) {
-          Request *r = slot->reqs[i];
-          std::memcpy(r->output, slot->host_out + slot->out_offsets[i],
-                      (uint64_t)r->length * gen->samples * 8);
-        }
-      }
-      const Clock::time_point now = Clock::now();
-      lock.lock();
-      inflight.pop_front();
-      for (Request *r : slot->reqs) {
-        r->status = st;
-        r->done = true;
-        r->latency_ms = std::chrono::duration<double, std::milli>(now - r->submitted).count();
-        if (st == mhcudaSuccess) {
-          latencies.push_back(r->latency_ms);
-          units += r->nnz * gen->samples;
-        }
-      }
-      last_done = now;
-      slot->reqs.clear();
-      slot->busy = false;
-      cv_work.notify_one();
-      cv_done.notify_all();
-    }
+    for (auto &s : slots) s->cv.notify_one();
   }
 };
 
@@ -273,8 +306,11 @@ MHCUDAServer *mhcuda_serve_start(MinhashCudaGenerator *gen, const MHCUDAServeCon
     srv->slots.push_back(std::move(s));
   }
   MHCUDAServer *p = srv.get();
+  for (auto &s : srv->slots) {
+    Slot *sl = s.get();
+    sl->worker = std::thread([p, sl] { p->worker_loop(sl); });
+  }
   srv->scheduler = std::thread([p] { p->schedule_loop(); });
-  srv->completer = std::thread([p] { p->complete_loop(); });
   return srv.release();
 }
 
@@ -367,13 +403,12 @@ MHCUDAResult mhcuda_serve_stop(MHCUDAServer *srv) {
     srv->cv_work.notify_all();
   }
   srv->scheduler.join();
-  srv->completer.join();
+  for (auto &s : srv->slots) s->worker.join();
   (void)hipSetDevice(srv->gen->device);
   for (auto &s : srv->slots) {
     (void)hipStreamSynchronize(s->stream);
     (void)hipEventDestroy(s->finished);
     (void)hipStreamDestroy(s->stream);
-    if (s->host_out) (void)hipHostFree(s->host_out);
   }
   delete srv;
   return mhcudaSuccess;

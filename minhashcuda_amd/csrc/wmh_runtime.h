@@ -82,6 +82,7 @@ struct MinhashCudaGenerator_ {
   uint64_t piece_nnz = 1ull << 28;
   uint64_t piece_rows = 1ull << 20;
   wmh::Tuning tuning{2, 2048, 16};
+  bool seg_auto = true;  // shrink the item size of small batches (host-known nnz)
   int kernel = mhcudaKernelAuto;
   // optional timing of the hashing kernel(s) of the last call
   bool timing = false;
@@ -99,7 +100,9 @@ namespace wmh {
 // Enqueues the chosen hashing kernel over device-resident matrices on `s`,
 // staging persistent-launch descriptors through `ctx`.  The caller owns the
 // synchronization of `ctx` with other streams.
+// `nnz_hint` (0: unknown) is the batch's total nonzero count when the host
+// knows it; it lets small batches use smaller work items.
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx);
+                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint);
 
 }  // namespace wmh

@@ -49,6 +49,20 @@ class Server:
             self._keep[t.value] = (w, c, r, out)
         return t.value
 
+    def submit_arrays(self, weights, cols, indptr, output):
+        """Queues a request given as contiguous arrays (float32 weights, uint32
+        columns and indptr, uint32 output of shape (rows, samples, 2)); pinned
+        arrays are uploaded and downloaded without staging copies."""
+        for a in (weights, cols, indptr, output):
+            assert a.flags["C_CONTIGUOUS"]
+        t = ctypes.c_uint64(0)
+        capi_mod.check(self.lib.mhcuda_serve_submit(
+            self.handle, weights.ctypes.data, cols.ctypes.data, indptr.ctypes.data,
+            len(indptr) - 1, output.ctypes.data, ctypes.byref(t)), "mhcuda_serve_submit")
+        with self._lock:
+            self._keep[t.value] = (weights, cols, indptr, output)
+        return t.value
+
     def wait(self, ticket, with_latency=False):
         """Blocks until the request is hashed; returns its hashes (and latency ms)."""
         lat = ctypes.c_double(0)

@@ -90,3 +90,37 @@ def test_oversize_request_is_rejected(gen_params):
     assert server.wait(server.submit(small)).shape == (3, SAMPLES, 2)
     assert server.stats()["rejected"] == 1
     server.stop()
+
+
+def pinned_copy(a):
+    """A copy of `a` in pinned host memory (torch pin_memory), as numpy."""
+    import torch
+    t = torch.empty(a.shape, dtype={np.float32: torch.float32, np.uint32: torch.int32}[a.dtype.type],
+                    pin_memory=True)
+    view = t.numpy().view(a.dtype)
+    view[...] = a
+    return t, view
+
+
+def test_pinned_requests_skip_staging(gen_params):
+    """Requests whose arrays live in pinned host memory are DMA'd directly;
+    mixed with pageable ones in the same launches, all equal the oracle."""
+    gen, params = gen_params
+    reqs = make_requests(24, seed=31)
+    server = serving.Server(gen, memory_budget=64 << 20, slots=2, max_batch_requests=8)
+    keep, tickets = [], []
+    for i, m in enumerate(reqs):
+        if i % 2:
+            tickets.append(server.submit(m))
+            continue
+        arrs = []
+        for a in (m.data.astype(np.float32), m.indices.astype(np.uint32),
+                  m.indptr.astype(np.uint32), np.zeros((m.shape[0], SAMPLES, 2), np.uint32)):
+            t, v = pinned_copy(np.ascontiguousarray(a))
+            keep.append(t)
+            arrs.append(v)
+        tickets.append(server.submit_arrays(*arrs))
+    outs = [server.wait(t) for t in tickets]
+    server.stop()
+    for m, h in zip(reqs, outs):
+        assert np.array_equal(h, oracle.calc_omp(*params, m.data, m.indices, m.indptr, 16)[0])
