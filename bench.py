@@ -41,7 +41,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", default="batched", choices=["batched", "plain", "serve", "copy"],
+    p.add_argument("--workload", default="batched", choices=["batched", "plain", "serve", "large", "copy"],
                    help="copy: a plain 4 GiB device-to-device copy (calibrates counters)")
     p.add_argument("--qps", default="1000,3000,6000,9000,12000",
                    help="serve: comma-separated arrival rates (requests/s)")
@@ -279,6 +279,103 @@ def serve_bench(args, torch):
         "sweep": results}), flush=True)
 
 
+def large_bench(args, torch):
+    """1,000,000 x 12,000,000 CSR, 16,800 nonzeros per row (density 0.0014),
+    uniform columns, weights uniform in (0, 100], 128 samples, parameters
+    resident (18.4 GB).  Two figures:
+      * kernel: the persistent kernel on one device-resident 65,536-row chunk
+        (1.1e9 nonzeros) -- the roofline figure, reported as `value`;
+      * end to end: all 1,000,000 rows through mhcuda_calc_batch from a pinned
+        host copy of that chunk (16 matrices of <= 65,536 rows, the last one
+        partial), with double-buffered uploads overlapping the hashing --
+        PCIe-inclusive, reported beside it."""
+    import numpy as np
+    import scipy.sparse as sp
+    from minhashcuda_amd import workloads
+    dim, per_row, chunk_rows, total_rows = 12_000_000, 16_800, 65_536, 1_000_000
+    steps = args.steps or 3
+    engine = Engine(dim, 1, args)
+    cm = engine.capi_mod
+    stream = torch.cuda.current_stream()
+    log("large: generating one 65,536-row chunk on the GPU")
+    w, c, r = workloads.batched_csr_torch(np.full(chunk_rows, per_row), dim, seed=7,
+                                          weights="uniform100")
+    nnz = int(w.numel())
+    out = torch.empty((chunk_rows, SAMPLES, 2), dtype=torch.int32, device="cuda")
+    desc = (cm.Matrix * 1)(cm.Matrix(w.data_ptr(), c.data_ptr(), r.data_ptr(), chunk_rows,
+                                     out.data_ptr()))
+    torch.cuda.synchronize()
+    wall, kms = timed_steps(torch, lambda: engine.batch(desc, cm.KERNEL_PERSISTENT, stream),
+                            engine, steps, 1)
+    avg_ms = sum(kms) / len(kms)
+    alg = workloads.algorithmic_bytes(nnz, chunk_rows, SAMPLES)
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    # pinned host copy of the chunk, then the whole 1M rows end to end
+    log("large: pinned host copy")
+    hw = torch.empty(nnz, dtype=torch.float32, pin_memory=True)
+    hc = torch.empty(nnz, dtype=torch.int32, pin_memory=True)
+    hr = torch.empty(chunk_rows + 1, dtype=torch.int32, pin_memory=True)
+    hw.copy_(w)
+    hc.copy_(c)
+    hr.copy_(r)
+    hout = torch.empty((total_rows, SAMPLES, 2), dtype=torch.int32, pin_memory=True)
+    lengths = [chunk_rows] * (total_rows // chunk_rows)
+    if total_rows % chunk_rows:
+        lengths.append(total_rows % chunk_rows)
+    mats = (cm.Matrix * len(lengths))()
+    row0 = 0
+    for i, n in enumerate(lengths):
+        mats[i] = cm.Matrix(hw.data_ptr(), hc.data_ptr(), hr.data_ptr(), n,
+                            hout[row0:].data_ptr())
+        row0 += n
+    torch.cuda.synchronize()
+    ends = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        cm.check(engine.capi.mhcuda_calc_batch(engine.g, len(lengths), mats,
+                                               cm.KERNEL_PERSISTENT), "calc_batch")
+        ends.append(time.perf_counter() - t0)
+    rp = r.cpu().numpy().view(np.uint32).astype(np.int64)
+    e2e_nnz = sum(int(rp[n]) for n in lengths)
+    e2e_units = e2e_nnz * SAMPLES
+    # checks: the end-to-end hashes of every chunk equal the resident chunk's
+    dev_h = out.cpu().numpy().view(np.uint32)
+    host_h = hout.numpy().view(np.uint32)
+    row0, same = 0, True
+    for n in lengths:
+        same = same and np.array_equal(host_h[row0:row0 + n], dev_h[:n])
+        row0 += n
+    result = {
+        "metric": METRIC, "workload": "large", "unit": "nnz*samples/s",
+        "value": nnz * SAMPLES * steps / wall, "steps": steps, "warmup": 1,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "n_gpus": 1,
+        "dtype": "f32", "data": "synthetic", "vs_baseline": None,
+        "config": dict({"workload": "large: 1,000,000 x 12,000,000 CSR, 16,800 nnz/row, "
+                                    "uniform(0,100] weights, 128 samples, parameters resident "
+                                    "(18.4 GB); value = persistent kernel on one resident "
+                                    "65,536-row chunk (1.1e9 nnz)",
+                        "chunk_rows": chunk_rows, "chunk_nnz": nnz}, **engine.tuning()),
+        "roofline": {"bound": "hbm", "kernel": "wmh_persistent_kernel", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("large", "wmh_persistent_kernel", alg),
+                     "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms},
+        "end_to_end": {"rows": total_rows, "nnz": e2e_nnz,
+                       "seconds": min(ends), "value": e2e_units / min(ends),
+                       "unit": "nnz*samples/s (PCIe-inclusive, pinned host input)",
+                       "identical_to_resident_chunk": bool(same)},
+    }
+    if not args.no_cpu_baseline:
+        log("large: cpu baseline (retrieving 18.4 GB of parameters)")
+        params = engine.L.minhash_cuda_retrieve_vars(engine.gen)
+        cut = 600
+        m0 = sp.csr_matrix((hw[:rp[cut]].numpy(), hc[:rp[cut]].numpy().view(np.uint32),
+                            rp[:cut + 1]), shape=(cut, dim))
+        result["cpu_baseline"] = cpu_baseline(params, m0, SAMPLES, args.cpu_seconds,
+                                              dev_h[:cut], "first rows of the chunk")
+    engine.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -307,6 +404,9 @@ def main():
         return
     if args.workload == "serve":
         serve_bench(args, torch)
+        return
+    if args.workload == "large":
+        large_bench(args, torch)
         return
     dim = 1 << 20
     engine = Engine(dim, 1, args)

@@ -461,11 +461,17 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     if (x.length > r0) pieces.back().push_back(Span{m, r0, x.length});
   }
   const size_t S2 = (size_t)gen->samples * 2;
+  for (auto &b : gen->pieces) {
+    // copies still owed here belong to a call that failed midway: drop them
+    HIPCH(hipEventSynchronize(b.done), mhcudaRuntimeError);
+    b.owed.clear();
+  }
   for (size_t p = 0; p < pieces.size(); p++) {
     const int b = (int)(p & 1);
     wmh::PieceBuffers &buf = gen->pieces[b];
     // the set is free once its previous download has completed
     HIPCH(hipEventSynchronize(buf.done), mhcudaRuntimeError);
+    buf.settle();
     uint64_t nnz = 0, nrows = 0;
     for (const Span &sp : pieces[p]) {
       nnz += mats[sp.m].rows[sp.r1] - mats[sp.m].rows[sp.r0];
@@ -508,16 +514,34 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     if (r != mhcudaSuccess) return r;
     HIPCH(hipEventRecord(buf.hashed, gen->stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->d2h_stream, buf.hashed, 0), mhcudaRuntimeError);
+    // pinned outputs are written directly; pageable ones through the landing zone
+    std::vector<char> direct(nspans);
+    size_t land = 0;
     for (size_t i = 0; i < nspans; i++) {
       const Span &sp = pieces[p][i];
-      HIPCH(hipMemcpyAsync(mats[sp.m].output + (uint64_t)sp.r0 * S2, dev[i].output,
-                           (uint64_t)(sp.r1 - sp.r0) * S2 * 4, hipMemcpyDeviceToHost,
+      direct[i] = wmh::is_pinned(mats[sp.m].output + (uint64_t)sp.r0 * S2);
+      if (!direct[i]) land += (uint64_t)(sp.r1 - sp.r0) * S2 * 4;
+    }
+    if (!buf.landing.reserve(land)) return mhcudaMemoryAllocationFailure;
+    land = 0;
+    for (size_t i = 0; i < nspans; i++) {
+      const Span &sp = pieces[p][i];
+      uint32_t *dst = mats[sp.m].output + (uint64_t)sp.r0 * S2;
+      const size_t bytes = (uint64_t)(sp.r1 - sp.r0) * S2 * 4;
+      void *target = dst;
+      if (!direct[i]) {
+        target = buf.landing.ptr + land;
+        buf.owed.push_back(wmh::PendingCopy{dst, target, bytes});
+        land += bytes;
+      }
+      HIPCH(hipMemcpyAsync(target, dev[i].output, bytes, hipMemcpyDeviceToHost,
                            gen->d2h_stream), mhcudaMemoryCopyError);
     }
     HIPCH(hipEventRecord(buf.done, gen->d2h_stream), mhcudaRuntimeError);
   }
   HIPCH(hipStreamSynchronize(gen->d2h_stream), mhcudaRuntimeError);
   HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
+  for (auto &b : gen->pieces) b.settle();
   INFO("mhcuda - success\n");
   return mhcudaSuccess;
 }

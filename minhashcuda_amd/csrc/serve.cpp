@@ -48,33 +48,6 @@ struct Request {
   double latency_ms = 0;
 };
 
-// Grow-only pinned host buffer.
-struct PinnedBuf {
-  char *ptr = nullptr;
-  size_t cap = 0;
-  bool reserve(size_t bytes) {
-    if (bytes <= cap) return true;
-    if (ptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-    if (hipHostMalloc((void **)&ptr, bytes, hipHostMallocDefault) != hipSuccess) return false;
-    cap = bytes;
-    return true;
-  }
-  ~PinnedBuf() {
-    if (ptr) (void)hipHostFree(ptr);
-  }
-};
-
-bool is_pinned(const void *p) {
-  hipPointerAttribute_t attr;
-  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-    (void)hipGetLastError();  // pageable memory is unknown to the runtime
-    return false;
-  }
-  return attr.type == hipMemoryTypeHost;
-}
-
 // One launch in flight: its own stream, device buffer, descriptor staging,
 // pinned landing zones and worker thread.  The worker uploads the admitted
 // requests, enqueues their hashing and downloads, waits for them and hands
@@ -85,8 +58,8 @@ struct Slot {
   hipEvent_t finished = nullptr;
   wmh::DevBuf buffer;
   wmh::PlanCtx plan;
-  PinnedBuf host_in;    // staging of pageable inputs
-  PinnedBuf host_out;   // landing zone of hashes bound for pageable outputs
+  wmh::PinnedBuf host_in;    // staging of pageable inputs
+  wmh::PinnedBuf host_out;   // landing zone of hashes bound for pageable outputs
   std::vector<Request *> reqs;
   bool busy = false;      // requests assigned and not yet retired
   std::thread worker;
@@ -133,13 +106,13 @@ struct MHCUDAServer_ {
     for (size_t i = 0; i < slot->reqs.size(); i++) {
       Request *r = slot->reqs[i];
       const uint32_t first = r->rows[0];
-      in_pinned[3 * i] = r->nnz == 0 || is_pinned(r->weights + first);
-      in_pinned[3 * i + 1] = r->nnz == 0 || is_pinned(r->cols + first);
-      in_pinned[3 * i + 2] = is_pinned(r->rows);
+      in_pinned[3 * i] = r->nnz == 0 || wmh::is_pinned(r->weights + first);
+      in_pinned[3 * i + 1] = r->nnz == 0 || wmh::is_pinned(r->cols + first);
+      in_pinned[3 * i + 2] = wmh::is_pinned(r->rows);
       if (!in_pinned[3 * i]) in_bytes += align256(r->nnz * 4);
       if (!in_pinned[3 * i + 1]) in_bytes += align256(r->nnz * 4);
       if (!in_pinned[3 * i + 2]) in_bytes += align256(((uint64_t)r->length + 1) * 4);
-      out_pinned[i] = is_pinned(r->output);
+      out_pinned[i] = wmh::is_pinned(r->output);
       if (!out_pinned[i]) out_bytes += align256(r->length * S8);
     }
     if (!slot->host_in.reserve(in_bytes) || !slot->host_out.reserve(out_bytes))

@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -27,10 +28,54 @@ struct DevBuf {
   hipError_t reserve(size_t bytes);
 };
 
+// Grow-only pinned host buffer.
+struct PinnedBuf {
+  char *ptr = nullptr;
+  size_t cap = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  bool reserve(size_t bytes) {
+    if (bytes <= cap) return true;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    if (hipHostMalloc((void **)&ptr, bytes, hipHostMallocDefault) != hipSuccess) return false;
+    cap = bytes;
+    return true;
+  }
+  ~PinnedBuf() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+};
+
+// True if `p` lies in pinned host memory known to the HIP runtime (then
+// copies to and from it are DMA'd asynchronously, without staging).
+inline bool is_pinned(const void *p) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is unknown to the runtime
+    return false;
+  }
+  return attr.type == hipMemoryTypeHost;
+}
+
+// A copy still owed from a pinned landing zone to a caller's output array.
+struct PendingCopy {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+
 // One device buffer set of the pipelined host-memory path, with the events
-// that order its upload, hashing and download.
+// that order its upload, hashing and download.  Hashes bound for pageable
+// outputs land in `landing` (a download into pageable memory would block the
+// host until the hashing finished) and are copied out when the set is next
+// reused or the call ends.
 struct PieceBuffers {
   DevBuf weights, cols, rows, hashes;
+  PinnedBuf landing;
+  std::vector<PendingCopy> owed;
   hipEvent_t uploaded = nullptr, hashed = nullptr, done = nullptr;
   hipError_t create() {
     hipError_t e = hipEventCreateWithFlags(&uploaded, hipEventDisableTiming);
@@ -43,6 +88,12 @@ struct PieceBuffers {
     if (hashed) (void)hipEventDestroy(hashed);
     if (done) (void)hipEventDestroy(done);
     uploaded = hashed = done = nullptr;
+  }
+  // Completes the copies owed by the previous use of this set (its `done`
+  // event must have completed).
+  void settle() {
+    for (const PendingCopy &c : owed) std::memcpy(c.dst, c.src, c.bytes);
+    owed.clear();
   }
 };
 

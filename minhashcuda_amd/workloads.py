@@ -56,10 +56,13 @@ def batched_csr(nnz_per_row, dim=1 << 20, sigma=1.5, seed=1):
     return sp.csr_matrix((w, cols, indptr.astype(np.uint32)), shape=(len(nnz_per_row), dim))
 
 
-def batched_csr_torch(nnz_per_row, dim=1 << 20, sigma=1.5, seed=1, device="cuda"):
-    """One matrix of the uneven batch generated directly in device memory with
-    torch (the full batch holds ~7e9 nonzeros, too many to build on the host).
-    Returns (weights f32, cols i32 as u32 bits, indptr i32 as u32 bits)."""
+def batched_csr_torch(nnz_per_row, dim=1 << 20, sigma=1.5, seed=1, device="cuda",
+                      weights="lognormal"):
+    """One CSR matrix generated directly in device memory with torch (the
+    benchmark batches hold billions of nonzeros, too many to build on the
+    host).  weights: "lognormal" (lognormal(0, sigma)) or "uniform100"
+    (uniform in (0, 100]).  Returns (weights f32, cols i32 holding u32 bits,
+    indptr i32 holding u32 bits)."""
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(int(seed))
@@ -69,7 +72,7 @@ def batched_csr_torch(nnz_per_row, dim=1 << 20, sigma=1.5, seed=1, device="cuda"
     total = int(indptr[-1].item())
     assert total < 2 ** 32
     cols = torch.empty(total, dtype=torch.int32, device=device)
-    weights = torch.empty(total, dtype=torch.float32, device=device)
+    vals = torch.empty(total, dtype=torch.float32, device=device)
     step = 1 << 27
     row_of = torch.repeat_interleave(torch.arange(len(nnz_per_row), device=device), nnz)
     for s in range(0, total, step):
@@ -81,10 +84,14 @@ def batched_csr_torch(nnz_per_row, dim=1 << 20, sigma=1.5, seed=1, device="cuda"
         hi = (pos + 1) * dim // n
         u = torch.rand(e - s, generator=g, device=device, dtype=torch.float64)
         cols[s:e] = (lo + (u * (hi - lo).to(torch.float64)).to(torch.int64)).to(torch.int32)
-        z = torch.randn(e - s, generator=g, device=device, dtype=torch.float32)
-        weights[s:e] = torch.exp(z * sigma)
+        if weights == "uniform100":
+            u2 = torch.rand(e - s, generator=g, device=device, dtype=torch.float32)
+            vals[s:e] = (1.0 - u2) * 100.0
+        else:
+            z = torch.randn(e - s, generator=g, device=device, dtype=torch.float32)
+            vals[s:e] = torch.exp(z * sigma)
     del row_of
-    return weights, cols, indptr.to(torch.int64).to(torch.int32)
+    return vals, cols, indptr.to(torch.int64).to(torch.int32)
 
 
 def algorithmic_bytes(nnz, rows, samples):
