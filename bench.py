@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--seg", type=int, default=0, help="nonzeros per work item (0: default)")
     p.add_argument("--wpc", type=int, default=0, help="persistent waves per CU (0: default)")
     p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
+    p.add_argument("--no-graph", action="store_true",
+                   help="launch eagerly instead of replaying a captured hipGraph")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target duration of the single-thread CPU oracle sample")
@@ -118,11 +120,14 @@ class Engine:
         self.g = ctypes.c_void_p(self.gen)
         capi_mod.check(self.capi.mhcuda_set_tuning(self.g, args.spl, args.seg, args.wpc), "tuning")
         capi_mod.check(self.capi.mhcuda_set_timing(self.g, 1), "timing")
+        capi_mod.check(self.capi.mhcuda_set_graph(self.g, 0 if args.no_graph else 1), "graph")
+        self.graph = not args.no_graph
 
     def tuning(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self.capi.mhcuda_get_tuning(self.g, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
-        return {"samples_per_lane": a.value, "segment": b.value, "waves_per_cu": c.value}
+        return {"samples_per_lane": a.value, "segment": b.value, "waves_per_cu": c.value,
+                "hip_graph": self.graph}
 
     def kernel_ms(self):
         ms = ctypes.c_float()
@@ -467,7 +472,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, kname, alg),
-                     "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms},
+                     "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms,
+                     "timed_span": ("hipGraph replay: descriptor upload, plan, init, %s, "
+                                    "fix-up (HIP events around the graph launch)" % kname
+                                    if engine.graph and kernel == cm.KERNEL_PERSISTENT else
+                                    "HIP events around the %s launch(es)" % kname)},
     })
     if args.workload == "batched" and not args.no_plain:
         ref_out = [o.clone() for o in outs]

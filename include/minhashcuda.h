@@ -139,8 +139,18 @@ MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen,
                                int *samples_per_lane, int *segment,
                                int *waves_per_cu);
 
+/* Enables (1) or disables (0) hipGraph replay of mhcuda_calc_device and
+ * mhcuda_calc_batch_device: the first call with a given set of descriptors,
+ * kernel and tuning is captured into a graph, later identical calls replay
+ * it with a single launch. */
+MHCUDAResult mhcuda_set_graph(MinhashCudaGenerator *gen, int enable);
+
+/* Number of graphs captured so far (each distinct call shape captures once). */
+MHCUDAResult mhcuda_graph_captures(const MinhashCudaGenerator *gen, uint64_t *captures);
+
 /* Enables (1) or disables (0) HIP-event timing of the hashing kernel(s) of
- * each call: the persistent kernel alone, or the span of the plain launches. */
+ * each call: the persistent kernel alone, or the span of the plain launches;
+ * for a graph replay, the whole replayed sequence. */
 MHCUDAResult mhcuda_set_timing(MinhashCudaGenerator *gen, int enable);
 
 /* Milliseconds taken by the timed kernel(s) of the last hashing call (waits

@@ -22,7 +22,8 @@ EXPORTS = (
     "mhcuda_init", "mhcuda_get_parameters", "mhcuda_retrieve_random_vars",
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
-    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_timing", "mhcuda_kernel_time",
+    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_graph", "mhcuda_graph_captures",
+    "mhcuda_set_timing", "mhcuda_kernel_time",
     "mhcuda_serve_start", "mhcuda_serve_submit", "mhcuda_serve_wait", "mhcuda_serve_stats",
     "mhcuda_serve_stop",
 )
@@ -85,6 +86,10 @@ def load(path):
     L.mhcuda_get_tuning.restype = i32
     L.mhcuda_get_tuning.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                     ctypes.POINTER(i32)]
+    L.mhcuda_set_graph.restype = i32
+    L.mhcuda_set_graph.argtypes = [vp, i32]
+    L.mhcuda_graph_captures.restype = i32
+    L.mhcuda_graph_captures.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.mhcuda_set_timing.restype = i32
     L.mhcuda_set_timing.argtypes = [vp, i32]
     L.mhcuda_kernel_time.restype = i32

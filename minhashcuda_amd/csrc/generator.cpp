@@ -156,7 +156,8 @@ uint32_t item_size(const MinhashCudaGenerator *gen, uint64_t nnz) {
 }
 
 static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                                    hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint) {
+                                    hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint,
+                                    bool capturing) {
   int verbosity = gen->verbosity;
   const uint32_t count = (uint32_t)mats.size();
   uint64_t chunks = 0;
@@ -168,9 +169,13 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   if (chunks >= (1ull << 32)) return mhcudaInvalidArguments;
   const size_t desc_bytes = sizeof(MatrixDesc) * count;
   const size_t ctl_off = (desc_bytes + 255) & ~(size_t)255;
-  if (!ctx->copied)
-    HIPCH(hipEventCreateWithFlags(&ctx->copied, hipEventDisableTiming), mhcudaRuntimeError);
-  HIPCH(hipEventSynchronize(ctx->copied), mhcudaRuntimeError);
+  // While a graph is being captured nothing may allocate or wait here: the
+  // graph's own staging is reserved and idle before capture starts.
+  if (!capturing) {
+    if (!ctx->copied)
+      HIPCH(hipEventCreateWithFlags(&ctx->copied, hipEventDisableTiming), mhcudaRuntimeError);
+    HIPCH(hipEventSynchronize(ctx->copied), mhcudaRuntimeError);
+  }
   HIPCH(ctx->plan.reserve(ctl_off + sizeof(BatchCtl)), mhcudaMemoryAllocationFailure);
   if (ctx->cap < count) {
     if (ctx->desc_host) (void)hipHostFree(ctx->desc_host);
@@ -184,13 +189,13 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   char *dev = (char *)ctx->plan.ptr;
   HIPCH(hipMemcpyAsync(dev, ctx->desc_host, desc_bytes, hipMemcpyHostToDevice, s),
         mhcudaMemoryCopyError);
-  HIPCH(hipEventRecord(ctx->copied, s), mhcudaRuntimeError);
+  if (!capturing) HIPCH(hipEventRecord(ctx->copied, s), mhcudaRuntimeError);
   Tuning tuning = gen->tuning;
   tuning.seg = (int)item_size(gen, nnz_hint);
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
                           (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), tuning,
-                          gen->num_cus, s, gen->timing ? gen->ev_begin : nullptr,
-                          gen->timing ? gen->ev_end : nullptr),
+                          gen->num_cus, s, gen->timing && !capturing ? gen->ev_begin : nullptr,
+                          gen->timing && !capturing ? gen->ev_end : nullptr),
         mhcudaRuntimeError);
   gen->timed = gen->timing;
   return mhcudaSuccess;
@@ -206,7 +211,8 @@ PlanCtx::~PlanCtx() {
 }
 
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint) {
+                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint,
+                               bool capturing) {
   int verbosity = gen->verbosity;
   if (gen->samples % 2 != 0) {
     // the reference finds no even sample_delta for odd counts
@@ -215,17 +221,95 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
   }
   if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
   if (kernel == mhcudaKernelAuto) kernel = mhcudaKernelPersistent;
-  if (kernel == mhcudaKernelPersistent) return hash_persistent(gen, mats, s, ctx, nnz_hint);
+  if (kernel == mhcudaKernelPersistent)
+    return hash_persistent(gen, mats, s, ctx, nnz_hint, capturing);
   if (kernel != mhcudaKernelPlain) return mhcudaInvalidArguments;
-  if (gen->timing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
+  if (gen->timing && !capturing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
   for (const MatrixDesc &m : mats) {
     if (m.length == 0) continue;
     HIPCH(launch_plain(gen->params, gen->dim, gen->samples, m, gen->tuning.spl, s),
           mhcudaRuntimeError);
   }
-  if (gen->timing) HIPCH(hipEventRecord(gen->ev_end, s), mhcudaRuntimeError);
+  if (gen->timing && !capturing) HIPCH(hipEventRecord(gen->ev_end, s), mhcudaRuntimeError);
   gen->timed = gen->timing;
   return mhcudaSuccess;
+}
+
+// hipGraph path of the device-pointer calls: the whole enqueue sequence
+// (descriptor upload, plan, init, hashing, fix-up -- or the plain launches)
+// is captured once per distinct (descriptors, kernel, tuning) and
+// replayed with one hipGraphLaunch.  The graph owns its descriptor staging,
+// so no other call can change what a replay copies; the sequence contains no
+// memset nodes.
+MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats, int kernel,
+                        hipStream_t s) {
+  int verbosity = gen->verbosity;
+  GraphCache &gc = gen->graph;
+  if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
+  if (kernel == mhcudaKernelAuto) kernel = mhcudaKernelPersistent;
+  const bool same = gc.exec && gc.kernel == kernel &&
+                    gc.tuning.spl == gen->tuning.spl && gc.tuning.seg == gen->tuning.seg &&
+                    gc.tuning.waves_per_cu == gen->tuning.waves_per_cu &&
+                    gc.key.size() == mats.size() &&
+                    std::memcmp(gc.key.data(), mats.data(), sizeof(MatrixDesc) * mats.size()) == 0;
+  if (!same) {
+    if (!gc.capture)
+      HIPCH(hipStreamCreateWithFlags(&gc.capture, hipStreamNonBlocking), mhcudaRuntimeError);
+    if (!gc.launched)
+      HIPCH(hipEventCreateWithFlags(&gc.launched, hipEventDisableTiming), mhcudaRuntimeError);
+    HIPCH(hipEventSynchronize(gc.launched), mhcudaRuntimeError);  // no replay in flight
+    gc.release();
+    // reserve the staging before capture: nothing allocates while capturing
+    const size_t desc_bytes = sizeof(MatrixDesc) * mats.size();
+    HIPCH(gc.plan.plan.reserve(((desc_bytes + 255) & ~(size_t)255) + sizeof(BatchCtl)),
+          mhcudaMemoryAllocationFailure);
+    if (gc.plan.cap < mats.size()) {
+      if (gc.plan.desc_host) (void)hipHostFree(gc.plan.desc_host);
+      gc.plan.desc_host = nullptr;
+      gc.plan.cap = 0;
+      HIPCH(hipHostMalloc((void **)&gc.plan.desc_host, desc_bytes, hipHostMallocDefault),
+            mhcudaMemoryAllocationFailure);
+      gc.plan.cap = mats.size();
+    }
+    std::vector<MatrixDesc> key = mats;
+    HIPCH(hipStreamBeginCapture(gc.capture, hipStreamCaptureModeRelaxed), mhcudaRuntimeError);
+    MHCUDAResult r = hash_device_batch(gen, mats, kernel, gc.capture, &gc.plan, 0, true);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(gc.capture, &graph);
+    if (r != mhcudaSuccess || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r != mhcudaSuccess ? r : mhcudaRuntimeError;
+    }
+    gc.graph = graph;
+    HIPCH(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0), mhcudaRuntimeError);
+    gc.key.swap(key);
+    gc.kernel = kernel;
+    gc.tuning = gen->tuning;
+    gc.captures++;
+  }
+  // timing events stay outside the graph: with graphs the timed span is the
+  // whole replayed sequence
+  if (gen->timing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
+  HIPCH(hipGraphLaunch(gc.exec, s), mhcudaRuntimeError);
+  if (gen->timing) HIPCH(hipEventRecord(gen->ev_end, s), mhcudaRuntimeError);
+  HIPCH(hipEventRecord(gc.launched, s), mhcudaRuntimeError);
+  gen->timed = gen->timing;
+  return mhcudaSuccess;
+}
+
+GraphCache::~GraphCache() {
+  if (launched) (void)hipEventSynchronize(launched);
+  release();
+  if (launched) (void)hipEventDestroy(launched);
+  if (capture) (void)hipStreamDestroy(capture);
+}
+
+void GraphCache::release() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  key.clear();
 }
 
 }  // namespace wmh
@@ -403,7 +487,9 @@ MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t
     descs[m] = wmh::MatrixDesc{mats[m].weights, mats[m].cols, mats[m].rows, mats[m].output,
                                mats[m].length, 0u, 0u, 0u, 0u, 0u};
   }
-  MHCUDAResult r = wmh::hash_device_batch(gen, descs, kernel, s, &gen->plan, 0);
+  MHCUDAResult r = gen->graph.enabled ? wmh::hash_graph(gen, descs, kernel, s)
+                                      : wmh::hash_device_batch(gen, descs, kernel, s,
+                                                               &gen->plan, 0, false);
   if (r != mhcudaSuccess) return r;
   if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
   return mhcudaSuccess;
@@ -510,7 +596,8 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     }
     HIPCH(hipEventRecord(buf.uploaded, gen->copy_stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->stream, buf.uploaded, 0), mhcudaRuntimeError);
-    MHCUDAResult r = wmh::hash_device_batch(gen, dev, kernel, gen->stream, &gen->plan, nnz);
+    MHCUDAResult r =
+        wmh::hash_device_batch(gen, dev, kernel, gen->stream, &gen->plan, nnz, false);
     if (r != mhcudaSuccess) return r;
     HIPCH(hipEventRecord(buf.hashed, gen->stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->d2h_stream, buf.hashed, 0), mhcudaRuntimeError);
@@ -588,6 +675,19 @@ MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *cgen, float *ms) {
   if (!gen->timed) return mhcudaInvalidArguments;
   HIPCH(hipEventSynchronize(gen->ev_end), mhcudaRuntimeError);
   HIPCH(hipEventElapsedTime(ms, gen->ev_begin, gen->ev_end), mhcudaRuntimeError);
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_set_graph(MinhashCudaGenerator *gen, int enable) {
+  if (!gen) return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  gen->graph.enabled = enable != 0;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_graph_captures(const MinhashCudaGenerator *gen, uint64_t *captures) {
+  if (!gen || !captures) return mhcudaInvalidArguments;
+  *captures = gen->graph.captures;
   return mhcudaSuccess;
 }
 

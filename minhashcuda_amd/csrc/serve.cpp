@@ -149,7 +149,7 @@ struct MHCUDAServer_ {
       total_nnz += r->nnz;
     }
     MHCUDAResult res = wmh::hash_device_batch(gen, descs, mhcudaKernelPersistent, slot->stream,
-                                              &slot->plan, total_nnz);
+                                              &slot->plan, total_nnz, false);
     if (res != mhcudaSuccess) return res;
     std::vector<size_t> out_off(slot->reqs.size());
     size_t landed = 0;

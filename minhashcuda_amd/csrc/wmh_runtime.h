@@ -111,6 +111,26 @@ struct PlanCtx {
   ~PlanCtx();
 };
 
+// Captured enqueue sequence of the device-pointer calls, replayed while the
+// call (descriptors, kernel, tuning) stays the same.
+struct GraphCache {
+  bool enabled = false;
+  std::vector<MatrixDesc> key;
+  int kernel = 0;
+  Tuning tuning{0, 0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture = nullptr;
+  hipEvent_t launched = nullptr;  // recorded after every replay
+  PlanCtx plan;
+  uint64_t captures = 0;
+  GraphCache() = default;
+  GraphCache(const GraphCache &) = delete;
+  GraphCache &operator=(const GraphCache &) = delete;
+  ~GraphCache();
+  void release();
+};
+
 }  // namespace wmh
 
 struct MinhashCudaGenerator_ {
@@ -134,6 +154,7 @@ struct MinhashCudaGenerator_ {
   uint64_t piece_rows = 1ull << 20;
   wmh::Tuning tuning{2, 2048, 16};
   bool seg_auto = true;  // shrink the item size of small batches (host-known nnz)
+  wmh::GraphCache graph;  // hipGraph replay of device-pointer calls (opt-in)
   int kernel = mhcudaKernelAuto;
   // optional timing of the hashing kernel(s) of the last call
   bool timing = false;
@@ -153,7 +174,14 @@ namespace wmh {
 // synchronization of `ctx` with other streams.
 // `nnz_hint` (0: unknown) is the batch's total nonzero count when the host
 // knows it; it lets small batches use smaller work items.
+// `capturing`: the stream is being captured into a graph, so nothing may
+// allocate or wait on the host (ctx's buffers must already be reserved).
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint);
+                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint,
+                               bool capturing);
+
+// The hipGraph-replayed form of hash_device_batch for device-pointer calls.
+MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats, int kernel,
+                        hipStream_t s);
 
 }  // namespace wmh
