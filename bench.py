@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
     p.add_argument("--no-graph", action="store_true",
                    help="launch eagerly instead of replaying a captured hipGraph")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="batched: keep the default tuning instead of autotuning in warm-up")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target duration of the single-thread CPU oracle sample")
@@ -455,6 +457,17 @@ def main():
     for i, ((w, c, r), o) in enumerate(zip(tensors, outs)):
         descs[i] = cm.Matrix(w.data_ptr(), c.data_ptr(), r.data_ptr(), lengths[i], o.data_ptr())
     torch.cuda.synchronize()
+    tuned = None
+    if args.workload == "batched" and not args.no_autotune and not (args.spl or args.seg or args.wpc):
+        t0 = time.perf_counter()
+        tr = cm.TuneResult()
+        cm.check(engine.capi.mhcuda_autotune(engine.g, len(descs), descs,
+                                             ctypes.c_void_p(stream.cuda_stream),
+                                             ctypes.byref(tr)), "autotune")
+        tuned = dict(tr.as_dict(), seconds=time.perf_counter() - t0)
+        kernel = tr.kernel
+        kname = "wmh_persistent_kernel" if kernel == cm.KERNEL_PERSISTENT else "wmh_plain_kernel"
+        log("autotuned: %s" % tuned)
     log("rows %d nnz %d; running %d warmup + %d timed steps" % (rows, nnz, warmup, steps))
 
     wall, kms = timed_steps(torch, lambda: engine.batch(descs, kernel, stream), engine, steps,
@@ -478,6 +491,8 @@ def main():
                                     if engine.graph and kernel == cm.KERNEL_PERSISTENT else
                                     "HIP events around the %s launch(es)" % kname)},
     })
+    if tuned is not None:
+        result["autotune"] = tuned
     if args.workload == "batched" and not args.no_plain:
         ref_out = [o.clone() for o in outs]
         for o in outs:

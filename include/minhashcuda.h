@@ -139,6 +139,31 @@ MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen,
                                int *samples_per_lane, int *segment,
                                int *waves_per_cu);
 
+/* Result of mhcuda_autotune. */
+typedef struct {
+  uint64_t key;              /* samples | p50 bin << 16 | p90 bin << 24 | size class << 32 */
+  uint64_t histogram_rows;   /* rows in the batch */
+  uint64_t histogram_nnz;    /* nonzeros in the batch */
+  int kernel;                /* chosen MHCUDAKernel */
+  int samples_per_lane;
+  int segment;
+  int waves_per_cu;
+  float sample_ms;           /* time of the chosen configuration on the timed prefix */
+  uint64_t sample_nnz;       /* nonzeros of the timed prefix */
+  int cached;                /* 1: taken from the cache, nothing was timed */
+} MHCUDATuneResult;
+
+/* Autotunes the kernel choice and tuning for a batch of device-resident
+ * matrices: measures the batch's nonzero-weighted row-length histogram on the
+ * device, and unless a batch of the same shape (histogram bins of the median
+ * and 90th-percentile nonzero, size class, samples) was tuned before, times
+ * candidates on a prefix of up to 2^28 nonzeros.  The choice becomes the
+ * generator's tuning (kernel mhcudaKernelAuto then uses it).  The outputs of
+ * the timed prefix are overwritten. */
+MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t count,
+                             const MHCUDAMatrix *mats, void *stream,
+                             MHCUDATuneResult *result);
+
 /* Enables (1) or disables (0) hipGraph replay of mhcuda_calc_device and
  * mhcuda_calc_batch_device: the first call with a given set of descriptors,
  * kernel and tuning is captured into a graph, later identical calls replay

@@ -38,6 +38,7 @@ SOURCES = [
     ("wmh_kernels.hip", True),
     ("generator.cpp", False),
     ("serve.cpp", False),
+    ("autotune.cpp", False),
     ("python_module.cpp", False),
 ]
 

@@ -22,7 +22,8 @@ EXPORTS = (
     "mhcuda_init", "mhcuda_get_parameters", "mhcuda_retrieve_random_vars",
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
-    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_graph", "mhcuda_graph_captures",
+    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_autotune", "mhcuda_set_graph",
+    "mhcuda_graph_captures",
     "mhcuda_set_timing", "mhcuda_kernel_time",
     "mhcuda_serve_start", "mhcuda_serve_submit", "mhcuda_serve_wait", "mhcuda_serve_stats",
     "mhcuda_serve_stop",
@@ -40,6 +41,18 @@ class Matrix(ctypes.Structure):
     _fields_ = [("weights", ctypes.c_void_p), ("cols", ctypes.c_void_p),
                 ("rows", ctypes.c_void_p), ("length", ctypes.c_uint32),
                 ("output", ctypes.c_void_p)]
+
+
+class TuneResult(ctypes.Structure):
+    """MHCUDATuneResult (minhashcuda.h)."""
+    _fields_ = [("key", ctypes.c_uint64), ("histogram_rows", ctypes.c_uint64),
+                ("histogram_nnz", ctypes.c_uint64), ("kernel", ctypes.c_int),
+                ("samples_per_lane", ctypes.c_int), ("segment", ctypes.c_int),
+                ("waves_per_cu", ctypes.c_int), ("sample_ms", ctypes.c_float),
+                ("sample_nnz", ctypes.c_uint64), ("cached", ctypes.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class ServeConfig(ctypes.Structure):
@@ -86,6 +99,8 @@ def load(path):
     L.mhcuda_get_tuning.restype = i32
     L.mhcuda_get_tuning.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                     ctypes.POINTER(i32)]
+    L.mhcuda_autotune.restype = i32
+    L.mhcuda_autotune.argtypes = [vp, u32, ctypes.POINTER(Matrix), vp, ctypes.POINTER(TuneResult)]
     L.mhcuda_set_graph.restype = i32
     L.mhcuda_set_graph.argtypes = [vp, i32]
     L.mhcuda_graph_captures.restype = i32

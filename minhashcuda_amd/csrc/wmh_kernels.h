@@ -76,4 +76,10 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
                              hipStream_t stream, hipEvent_t ev_begin = nullptr,
                              hipEvent_t ev_end = nullptr);
 
+// Row-length histogram of a batch (device MatrixDesc array with chunk_begin
+// filled in): hist[0..32] row counts and hist[33..65] nonzero sums by the bit
+// length of the row's nonzero count.  `hist` must be zeroed beforehand.
+hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
+                            unsigned long long *hist, hipStream_t stream);
+
 }  // namespace wmh

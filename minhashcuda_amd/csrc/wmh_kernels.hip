@@ -644,6 +644,35 @@ __global__ __launch_bounds__(kBlock) void batch_rows_kernel(
   }
 }
 
+// Row-length histogram of a batch for the autotuner: hist[b] counts rows and
+// hist[33 + b] sums nonzeros over rows whose nonzero count n has bit length
+// b (b = 0 for empty rows, 1 for n = 1, 2 for 2..3, ...).
+__global__ __launch_bounds__(kBlock) void batch_histogram_kernel(
+    const MatrixDesc *__restrict__ mats, uint32_t count, uint32_t total_chunks,
+    unsigned long long *__restrict__ hist) {
+  __shared__ unsigned long long local[66];
+  for (uint32_t i = threadIdx.x; i < 66; i += kBlock) local[i] = 0;
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  for (uint32_t w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); w < total_chunks;
+       w += gridDim.x * kWavesPerBlock) {
+    const uint32_t wv = uniform(w);
+    const uint32_t m = find_matrix<&MatrixDesc::chunk_begin>(mats, count, wv);
+    const MatrixDesc M = mats[m];
+    const uint32_t row = (wv - M.chunk_begin) * kWave + lane;
+    if (row < M.length) {
+      const uint32_t s = M.rows[row], e = M.rows[row + 1];
+      const uint32_t n = e >= s ? e - s : 0u;
+      const uint32_t b = n ? 32u - (uint32_t)__builtin_clz(n) : 0u;
+      atomicAdd(&local[b], 1ull);
+      atomicAdd(&local[33 + b], (unsigned long long)n);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 66; i += kBlock)
+    if (local[i]) atomicAdd(&hist[i], local[i]);
+}
+
 unsigned grid_for(uint64_t waves) {
   const uint64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
   return (unsigned)(blocks < (1u << 20) ? (blocks ? blocks : 1) : (1u << 20));
@@ -707,6 +736,14 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
     default:
       return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
+                            unsigned long long *hist, hipStream_t stream) {
+  const unsigned grid = std::min<unsigned>(grid_for(total_chunks), 1024u);
+  hipLaunchKernelGGL(batch_histogram_kernel, dim3(grid), dim3(kBlock), 0, stream, mats, count,
+                     total_chunks, hist);
   return hipGetLastError();
 }
 

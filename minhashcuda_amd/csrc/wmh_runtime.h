@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -111,6 +112,14 @@ struct PlanCtx {
   ~PlanCtx();
 };
 
+// A tuning chosen by the autotuner for one batch shape.
+struct TunedChoice {
+  int kernel = 0;
+  Tuning tuning{2, 2048, 16};
+  float ms = 0;           // time of the timed prefix
+  uint64_t sample_nnz = 0;
+};
+
 // Captured enqueue sequence of the device-pointer calls, replayed while the
 // call (descriptors, kernel, tuning) stays the same.
 struct GraphCache {
@@ -155,6 +164,8 @@ struct MinhashCudaGenerator_ {
   wmh::Tuning tuning{2, 2048, 16};
   bool seg_auto = true;  // shrink the item size of small batches (host-known nnz)
   wmh::GraphCache graph;  // hipGraph replay of device-pointer calls (opt-in)
+  std::map<uint64_t, wmh::TunedChoice> tuned;  // autotuner cache by batch shape
+  wmh::DevBuf tune_buf;
   int kernel = mhcudaKernelAuto;
   // optional timing of the hashing kernel(s) of the last call
   bool timing = false;

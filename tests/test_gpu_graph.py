@@ -68,3 +68,39 @@ def test_graph_replays_equal_eager(libmhcuda, capi, gpu_available, kernel):
         assert np.array_equal(b[3].cpu().numpy().view(np.uint32), e)
     assert not bufs[2][3].any()
     libmhcuda.minhash_cuda_fini(gen)
+
+
+def test_autotune_picks_and_caches(libmhcuda, capi, gpu_available):
+    """The autotuner measures the batch's row-length histogram, times
+    candidates once per batch shape, caches the choice and installs it; hashes
+    under the tuned choice still equal the oracle's."""
+    layout = workloads.batched_layout(count=4, min_rows=100, max_rows=800, seed=12)
+    mats = [workloads.batched_csr(n, dim=DIM, seed=60 + i) for i, n in enumerate(layout)]
+    gen = libmhcuda.minhash_cuda_init(DIM, 128, seed=6)
+    g = ctypes.c_void_p(gen)
+    params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+    bufs, descs = device_batch(mats)
+    sptr = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    r1 = capi_mod.TuneResult()
+    assert capi.mhcuda_autotune(g, 4, descs, sptr, ctypes.byref(r1)) == 0
+    assert r1.cached == 0 and r1.histogram_rows == sum(m.shape[0] for m in mats)
+    assert r1.histogram_nnz == sum(m.nnz for m in mats)
+    assert r1.kernel in (capi_mod.KERNEL_PLAIN, capi_mod.KERNEL_PERSISTENT)
+    assert r1.samples_per_lane in (1, 2, 4) and r1.sample_ms > 0
+    r2 = capi_mod.TuneResult()
+    assert capi.mhcuda_autotune(g, 4, descs, sptr, ctypes.byref(r2)) == 0
+    assert r2.cached == 1 and r2.key == r1.key
+    assert (r2.kernel, r2.samples_per_lane, r2.segment) == (r1.kernel, r1.samples_per_lane,
+                                                            r1.segment)
+    spl, seg, wpc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    capi.mhcuda_get_tuning(g, ctypes.byref(spl), ctypes.byref(seg), ctypes.byref(wpc))
+    assert (spl.value, seg.value, wpc.value) == (r1.samples_per_lane, r1.segment,
+                                                 r1.waves_per_cu)
+    for b in bufs:
+        b[3].zero_()
+    assert capi.mhcuda_calc_batch_device(g, 4, descs, capi_mod.KERNEL_AUTO, sptr) == 0
+    torch.cuda.synchronize()
+    for m, b in zip(mats, bufs):
+        ref = oracle.calc_omp(*params, m.data, m.indices, m.indptr, 16)[0]
+        assert np.array_equal(b[3].cpu().numpy().view(np.uint32), ref)
+    libmhcuda.minhash_cuda_fini(gen)
