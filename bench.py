@@ -41,7 +41,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", default="batched", choices=["batched", "plain", "serve", "large", "copy"],
+    p.add_argument("--workload", default="batched", choices=["batched", "plain", "serve", "large", "c1", "copy"],
                    help="copy: a plain 4 GiB device-to-device copy (calibrates counters)")
     p.add_argument("--qps", default="1000,3000,6000,9000,12000",
                    help="serve: comma-separated arrival rates (requests/s)")
@@ -286,6 +286,59 @@ def serve_bench(args, torch):
         "sweep": results}), flush=True)
 
 
+def c1_bench(args, torch):
+    """The reference's CPU-runnable case (test.py:93-97, README.md:85-90):
+    6400 x 130 CSR from numpy seed 1 (values in [0,100), 80% zeroed), 128
+    samples, parameters assigned through assign_random_vars.  Times the whole
+    6400 rows on the CPU oracle (single thread and OpenMP) and on the GPU
+    (device-resident input), and checks that the hashes agree."""
+    import numpy as np
+    import scipy.sparse as sp
+    import oracle
+    from minhashcuda_amd import capi as cm
+    data = oracle.c1_data(1)
+    m = sp.csr_matrix(data, dtype=np.float32)
+    rs, ln_cs, betas = oracle.datasketch_style_params(130, SAMPLES, seed=1)
+    steps = args.steps or 200
+    engine = Engine(130, 1, args)
+    engine.L.minhash_cuda_assign_vars(engine.gen, rs, ln_cs, betas)
+    stream = torch.cuda.current_stream()
+    w = torch.from_numpy(m.data.astype(np.float32)).cuda()
+    c = torch.from_numpy(m.indices.astype(np.int32)).cuda()
+    r = torch.from_numpy(m.indptr.astype(np.int64).astype(np.int32)).cuda()
+    out = torch.empty((m.shape[0], SAMPLES, 2), dtype=torch.int32, device="cuda")
+    desc = (cm.Matrix * 1)(cm.Matrix(w.data_ptr(), c.data_ptr(), r.data_ptr(), m.shape[0],
+                                     out.data_ptr()))
+    res = {}
+    for name, kernel in (("plain", cm.KERNEL_PLAIN), ("persistent", cm.KERNEL_PERSISTENT)):
+        wall, kms = timed_steps(torch, lambda: engine.batch(desc, kernel, stream), engine, steps, 5)
+        res[name] = {"value": m.nnz * SAMPLES * steps / wall, "ms_per_step": wall * 1e3 / steps,
+                     "avg_kernel_ms": sum(kms) / len(kms)}
+    gpu_h = out.cpu().numpy().view(np.uint32)
+    t = time.perf_counter()
+    h1 = oracle.calc_csr(rs, ln_cs, betas, m)
+    t1 = time.perf_counter() - t
+    threads = min(cpu_threads(), 16)
+    t = time.perf_counter()
+    h2, used = oracle.calc_omp(rs, ln_cs, betas, m.data, m.indices, m.indptr, threads)
+    t2 = time.perf_counter() - t
+    best = max(res, key=lambda k: res[k]["value"])
+    print(json.dumps({
+        "metric": METRIC, "workload": "c1", "unit": "nnz*samples/s", "value": res[best]["value"],
+        "steps": steps, "higher_is_better": True, "n_gpus": 1, "dtype": "f32",
+        "data": "synthetic (test.py recipe, seed 1)",
+        "config": {"workload": "c1: 6400 x 130 CSR (nnz %d), 128 samples, assigned parameters; "
+                               "GPU value = best of plain / persistent" % m.nnz},
+        "gpu": res,
+        "cpu_baseline": {"value": m.nnz * SAMPLES / t2, "unit": "nnz*samples/s", "cores": used,
+                         "kind": "port", "sample": "all 6400 rows, OpenMP C oracle",
+                         "single_thread_value": m.nnz * SAMPLES / t1,
+                         "single_thread_sample": "all 6400 rows, %.3f s" % t1},
+        "bit_exact": bool(np.array_equal(gpu_h, h1) and np.array_equal(h1, h2)),
+    }), flush=True)
+    engine.close()
+
+
 def large_bench(args, torch):
     """1,000,000 x 12,000,000 CSR, 16,800 nonzeros per row (density 0.0014),
     uniform columns, weights uniform in (0, 100], 128 samples, parameters
@@ -414,6 +467,9 @@ def main():
         return
     if args.workload == "large":
         large_bench(args, torch)
+        return
+    if args.workload == "c1":
+        c1_bench(args, torch)
         return
     dim = 1 << 20
     engine = Engine(dim, 1, args)

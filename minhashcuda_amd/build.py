@@ -18,8 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
-BUILD = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "libMHCUDA.so")
+BUILD = os.environ.get("MHCUDA_BUILD_DIR", os.path.join(HERE, "_build"))
+LIB = os.environ.get("MHCUDA_LIB_OUT", os.path.join(HERE, "libMHCUDA.so"))
 ARCH = os.environ.get("MHCUDA_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
@@ -27,7 +27,7 @@ COMMON = [
     "-ffp-contract=off", "-fno-fast-math",
     "-I" + INCLUDE, "-I" + CSRC,
 ]
-DEVICE = [
+DEVICE = os.environ.get("MHCUDA_EXTRA_DEVICE_FLAGS", "").split() + [
     "--offload-arch=" + ARCH,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -70,10 +70,10 @@ def build(verbose=False):
     objs = []
     for src, is_device in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), "-c", os.path.join(CSRC, src), "-o", obj] + COMMON
-        cmd += DEVICE if is_device else HOST
+        cmd = [hipcc(), "-c"] + COMMON + (DEVICE if is_device else HOST)
         if src == "python_module.cpp":
             cmd += ["-I" + py_inc, "-I" + numpy.get_include()]
+        cmd += [os.path.join(CSRC, src), "-o", obj]  # after -x: it applies to the file
         _run(cmd, verbose)
         objs.append(obj)
     libdir = sysconfig.get_config_var("LIBDIR")

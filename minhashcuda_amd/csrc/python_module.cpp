@@ -34,6 +34,7 @@ struct Ref {
   Ref() = default;
   explicit Ref(PyObject *o) : p(o) {}
   Ref(const Ref &) = delete;
+  Ref(Ref &&o) noexcept : p(o.p) { o.p = nullptr; }
   Ref &operator=(const Ref &) = delete;
   Ref &operator=(Ref &&o) noexcept {
     if (this != &o) {
@@ -92,6 +93,20 @@ struct Csr {
   uint32_t length = 0;
 };
 
+// `obj` as a C-contiguous array of uint32 bit patterns.  int32 and uint32
+// arrays are used in place (the force-cast of python.cc:317,323 would produce
+// the same bits with a copy); other integer types are force-cast.
+Ref as_u32(PyObject *obj) {
+  Ref arr(PyArray_FROM_OF(obj, NPY_ARRAY_IN_ARRAY));
+  if (arr) {
+    const int t = PyArray_TYPE(arr.arr());
+    if ((t == NPY_INT32 || t == NPY_UINT32) && PyArray_ITEMSIZE(arr.arr()) == 4) return arr;
+  } else {
+    PyErr_Clear();
+  }
+  return Ref(PyArray_FROM_OTF(obj, NPY_UINT32, NPY_ARRAY_FORCECAST | NPY_ARRAY_IN_ARRAY));
+}
+
 bool load_csr(PyObject *matrix, Csr *out) {
   Ref sparse(PyImport_ImportModule("scipy.sparse"));
   if (!sparse) {
@@ -117,8 +132,7 @@ bool load_csr(PyObject *matrix, Csr *out) {
   }
   Ref indices(PyObject_GetAttrString(matrix, "indices"));
   if (!indices) return false;
-  out->cols = Ref(PyArray_FROM_OTF(indices.get(), NPY_UINT32,
-                                   NPY_ARRAY_FORCECAST | NPY_ARRAY_IN_ARRAY));
+  out->cols = as_u32(indices.get());
   if (!out->cols) {
     PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.indices to numpy.array");
     return false;
@@ -127,23 +141,23 @@ bool load_csr(PyObject *matrix, Csr *out) {
   if (!indptr) return false;
   // the engine's indptr is uint32 (minhashcuda.h:117): larger matrices must be
   // hashed in row slices
-  Ref indptr64(PyArray_FROM_OTF(indptr.get(), NPY_INT64, NPY_ARRAY_FORCECAST | NPY_ARRAY_IN_ARRAY));
-  if (!indptr64) return false;
-  const npy_intp n = PyArray_SIZE(indptr64.arr());
-  if (n < 1) {
-    PyErr_SetString(PyExc_ValueError, "csr_matrix.indptr is empty");
-    return false;
-  }
-  const int64_t last = reinterpret_cast<const int64_t *>(PyArray_DATA(indptr64.arr()))[n - 1];
-  if (last > (int64_t)0xffffffffLL) {
+  Ref last_obj(PySequence_GetItem(indptr.get(), -1));
+  if (!last_obj) return false;
+  const long long last = PyLong_AsLongLong(last_obj.get());
+  if (last == -1 && PyErr_Occurred()) return false;
+  if (last > 0xffffffffLL) {
     PyErr_SetString(PyExc_ValueError,
                     "csr_matrix has 2^32 or more nonzeros: hash it in row slices");
     return false;
   }
-  out->rows = Ref(PyArray_FROM_OTF(indptr.get(), NPY_UINT32,
-                                   NPY_ARRAY_FORCECAST | NPY_ARRAY_IN_ARRAY));
+  out->rows = as_u32(indptr.get());
   if (!out->rows) {
     PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.indptr to numpy.array");
+    return false;
+  }
+  const npy_intp n = PyArray_SIZE(out->rows.arr());
+  if (n < 1) {
+    PyErr_SetString(PyExc_ValueError, "csr_matrix.indptr is empty");
     return false;
   }
   out->length = static_cast<uint32_t>(n - 1);

@@ -168,6 +168,13 @@ struct Argmin {
   }
 };
 
+// Nonzeros x samples-per-lane whose gathers are issued together per group
+// (16: 24 loads in flight per wavefront at 2 samples per lane; doubling it
+// measured no change on MI355X).
+#ifndef WMH_GROUP_NNZ
+#define WMH_GROUP_NNZ 16
+#endif
+
 // Loads V consecutive floats starting at p (8/16-byte aligned for V = 2/4).
 template <int V>
 __device__ __forceinline__ void load_v(const float *__restrict__ p, float (&out)[V]) {
@@ -325,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void wmh_plain_kernel(
     const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
     Argmin<V> acc;
     acc.init();
-    hash_range<V, 16 / V>(params, dim, 12ull * sp, lo, weights, cols, begin, end, acc);
+    hash_range<V, WMH_GROUP_NNZ / V>(params, dim, 12ull * sp, lo, weights, cols, begin, end, acc);
     if (k0 < samples) write_hashes<V>(output, row, samples, k0, acc);
   }
 }
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     const uint32_t k0 = tile * (kWave * V) + lane * V;
     const uint32_t koff = (k0 < samples) ? k0 : 0u;
     const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
-    hash_item<V, 16 / V>(params, dim, sp, samples, k0, lo, M, a, b);
+    hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b);
   }
 }
 
