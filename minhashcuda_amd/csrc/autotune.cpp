@@ -135,7 +135,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
       gen->tuning = c.tuning;
       std::vector<wmh::MatrixDesc> d = sample;
       if (hipEventRecord(e0, s) != hipSuccess) return mhcudaRuntimeError;
-      MHCUDAResult r = wmh::hash_device_batch(gen, d, c.kernel, s, &gen->plan, 0, false);
+      MHCUDAResult r = wmh::hash_device_batch(gen, d, c.kernel, s, &gen->plan, false);
       if (r != mhcudaSuccess) return r;
       if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
           hipEventElapsedTime(ms, e0, e1) != hipSuccess)

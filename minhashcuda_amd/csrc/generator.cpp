@@ -143,21 +143,8 @@ namespace wmh {
 
 // Enqueues the persistent batched path on `s`: descriptors go through the
 // pinned staging buffer into device memory next to the control block.
-// Item size for a batch of `nnz` nonzeros: the configured size, shrunk (in
-// steps of 64, down to 256) until the batch yields about four work items per
-// resident wavefront, so that small batches still spread over every CU.
-uint32_t item_size(const MinhashCudaGenerator *gen, uint64_t nnz) {
-  uint32_t seg = (uint32_t)gen->tuning.seg;
-  if (!gen->seg_auto || nnz == 0) return seg;
-  const uint64_t waves = (uint64_t)std::max(1, gen->num_cus) * gen->tuning.waves_per_cu;
-  const uint64_t want = nnz / (waves * 4);
-  if (want < seg) seg = (uint32_t)std::max<uint64_t>(256, want & ~(uint64_t)63);
-  return std::min<uint32_t>(seg, (uint32_t)gen->tuning.seg);
-}
-
 static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                                    hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint,
-                                    bool capturing) {
+                                    hipStream_t s, PlanCtx *ctx, bool capturing) {
   int verbosity = gen->verbosity;
   const uint32_t count = (uint32_t)mats.size();
   uint64_t chunks = 0;
@@ -190,10 +177,9 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   HIPCH(hipMemcpyAsync(dev, ctx->desc_host, desc_bytes, hipMemcpyHostToDevice, s),
         mhcudaMemoryCopyError);
   if (!capturing) HIPCH(hipEventRecord(ctx->copied, s), mhcudaRuntimeError);
-  Tuning tuning = gen->tuning;
-  tuning.seg = (int)item_size(gen, nnz_hint);
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
-                          (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), tuning,
+                          (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), gen->tuning,
+                          gen->seg_auto,
                           gen->num_cus, s, gen->timing && !capturing ? gen->ev_begin : nullptr,
                           gen->timing && !capturing ? gen->ev_end : nullptr),
         mhcudaRuntimeError);
@@ -211,8 +197,7 @@ PlanCtx::~PlanCtx() {
 }
 
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint,
-                               bool capturing) {
+                               int kernel, hipStream_t s, PlanCtx *ctx, bool capturing) {
   int verbosity = gen->verbosity;
   if (gen->samples % 2 != 0) {
     // the reference finds no even sample_delta for odd counts
@@ -222,7 +207,7 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
   if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
   if (kernel == mhcudaKernelAuto) kernel = mhcudaKernelPersistent;
   if (kernel == mhcudaKernelPersistent)
-    return hash_persistent(gen, mats, s, ctx, nnz_hint, capturing);
+    return hash_persistent(gen, mats, s, ctx, capturing);
   if (kernel != mhcudaKernelPlain) return mhcudaInvalidArguments;
   if (gen->timing && !capturing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
   for (const MatrixDesc &m : mats) {
@@ -273,7 +258,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
     }
     std::vector<MatrixDesc> key = mats;
     HIPCH(hipStreamBeginCapture(gc.capture, hipStreamCaptureModeRelaxed), mhcudaRuntimeError);
-    MHCUDAResult r = hash_device_batch(gen, mats, kernel, gc.capture, &gc.plan, 0, true);
+    MHCUDAResult r = hash_device_batch(gen, mats, kernel, gc.capture, &gc.plan, true);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(gc.capture, &graph);
     if (r != mhcudaSuccess || ec != hipSuccess) {
@@ -489,7 +474,7 @@ MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t
   }
   MHCUDAResult r = gen->graph.enabled ? wmh::hash_graph(gen, descs, kernel, s)
                                       : wmh::hash_device_batch(gen, descs, kernel, s,
-                                                               &gen->plan, 0, false);
+                                                               &gen->plan, false);
   if (r != mhcudaSuccess) return r;
   if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
   return mhcudaSuccess;
@@ -546,6 +531,11 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     }
     if (x.length > r0) pieces.back().push_back(Span{m, r0, x.length});
   }
+  // Default kernel for host-memory calls when neither the caller nor the
+  // autotuner chose one: one wavefront per row suits a single matrix of
+  // bounded rows (no stragglers); anything else goes to the persistent queue.
+  if (kernel == mhcudaKernelAuto && gen->kernel == mhcudaKernelAuto)
+    kernel = (count == 1 && max_row <= 4096) ? mhcudaKernelPlain : mhcudaKernelPersistent;
   const size_t S2 = (size_t)gen->samples * 2;
   for (auto &b : gen->pieces) {
     // copies still owed here belong to a call that failed midway: drop them
@@ -597,7 +587,7 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
     HIPCH(hipEventRecord(buf.uploaded, gen->copy_stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->stream, buf.uploaded, 0), mhcudaRuntimeError);
     MHCUDAResult r =
-        wmh::hash_device_batch(gen, dev, kernel, gen->stream, &gen->plan, nnz, false);
+        wmh::hash_device_batch(gen, dev, kernel, gen->stream, &gen->plan, false);
     if (r != mhcudaSuccess) return r;
     HIPCH(hipEventRecord(buf.hashed, gen->stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->d2h_stream, buf.hashed, 0), mhcudaRuntimeError);

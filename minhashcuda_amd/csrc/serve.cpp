@@ -125,7 +125,7 @@ struct MHCUDAServer_ {
       }
       return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, slot->stream) == hipSuccess;
     };
-    uint64_t off = 0, total_nnz = 0;
+    uint64_t off = 0;
     size_t staged = 0;
     std::vector<wmh::MatrixDesc> descs;
     descs.reserve(slot->reqs.size());
@@ -146,10 +146,9 @@ struct MHCUDAServer_ {
       if (!upload(dr, r->rows, ((uint64_t)r->length + 1) * 4, in_pinned[3 * i + 2], staged))
         return mhcudaMemoryCopyError;
       descs.push_back(wmh::MatrixDesc{dw, dc, dr, dh, r->length, first, 0u, 0u, 0u, 0u});
-      total_nnz += r->nnz;
     }
     MHCUDAResult res = wmh::hash_device_batch(gen, descs, mhcudaKernelPersistent, slot->stream,
-                                              &slot->plan, total_nnz, false);
+                                              &slot->plan, false);
     if (res != mhcudaSuccess) return res;
     std::vector<size_t> out_off(slot->reqs.size());
     size_t landed = 0;

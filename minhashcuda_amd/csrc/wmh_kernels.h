@@ -36,7 +36,8 @@ struct BatchCtl {
   uint32_t total_items;
   uint32_t ntiles;
   uint32_t invalid;       // nonzero: some indptr decreases; nothing is hashed
-  uint32_t pad_[3];
+  uint32_t seg;           // item size chosen by the plan kernel
+  uint32_t pad_[2];
 };
 
 // Kernel tuning knobs (chosen by the autotuner).
@@ -68,12 +69,14 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
 // plan, init and fix-up passes).  `mats` and `ctl` are device memory; the
 // host fills every MatrixDesc field except item_begin / n_items, and
 // chunk_begin must be the exclusive prefix of ceil(length / 64).
+// tuning.seg is the item size, or with `auto_seg` its upper bound (the plan
+// kernel then sizes items for about four per resident wavefront).
 // `ev_begin` / `ev_end` (may be null) are recorded right around the
 // persistent hashing kernel.
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
-                             BatchCtl *ctl, const Tuning &tuning, int num_cus,
-                             hipStream_t stream, hipEvent_t ev_begin = nullptr,
+                             BatchCtl *ctl, const Tuning &tuning, bool auto_seg,
+                             int num_cus, hipStream_t stream, hipEvent_t ev_begin = nullptr,
                              hipEvent_t ev_end = nullptr);
 
 // Row-length histogram of a batch (device MatrixDesc array with chunk_begin

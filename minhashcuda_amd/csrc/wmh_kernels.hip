@@ -520,10 +520,10 @@ __device__ __forceinline__ uint32_t find_matrix(const MatrixDesc *__restrict__ m
 template <int V>
 __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
-    const MatrixDesc *__restrict__ mats, uint32_t count, uint32_t seg,
-    BatchCtl *__restrict__ ctl) {
+    const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl) {
   const uint32_t lane = lane_id();
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
+  const uint32_t seg = uniform(ctl->seg);
   const uint32_t total = uniform(ctl->total_units);
   const uint32_t ntiles = uniform(ctl->ntiles);
   for (;;) {
@@ -546,28 +546,48 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   }
 }
 
-// Item prefix over the matrices (one workgroup) and reset of the work queue.
+// Item plan (one workgroup): the item size, the item prefix over the
+// matrices, and the reset of the work queue.  With `target_items` > 0 the
+// item size is the batch's nonzeros / target_items rounded down to a multiple
+// of 64, within [64, max_seg], so that a small batch still yields enough
+// items to occupy every CU; otherwise it is max_seg.
 __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restrict__ mats,
-                                                            uint32_t count, uint32_t seg,
+                                                            uint32_t count, uint32_t max_seg,
+                                                            uint32_t target_items,
                                                             uint32_t ntiles,
                                                             BatchCtl *__restrict__ ctl) {
   __shared__ uint32_t part[kBlock];
   __shared__ uint32_t carry;
-  const uint32_t tid = threadIdx.x;
   __shared__ uint32_t bad;
+  __shared__ unsigned long long total;
+  const uint32_t tid = threadIdx.x;
   if (tid == 0) {
     carry = 0;
     bad = 0;
+    total = 0;
   }
   __syncthreads();
+  // pass 1: total nonzeros (and indptr sanity)
+  for (uint32_t m = tid; m < count; m += kBlock) {
+    if (mats[m].length > 0) {
+      const uint32_t first = mats[m].rows[0], last = mats[m].rows[mats[m].length];
+      if (last < first) bad = 1;
+      else atomicAdd(&total, (unsigned long long)(last - first));
+    }
+  }
+  __syncthreads();
+  uint32_t seg = max_seg;
+  if (target_items > 0) {
+    const unsigned long long want = (total / target_items) & ~63ull;
+    seg = (uint32_t)min((unsigned long long)max_seg, max(64ull, want));
+  }
+  // pass 2: item prefix
   for (uint32_t base = 0; base < count; base += kBlock) {
     const uint32_t m = base + tid;
     uint32_t n = 0;
     if (m < count && mats[m].length > 0) {
       const uint32_t first = mats[m].rows[0], last = mats[m].rows[mats[m].length];
-      if (last < first) {
-        bad = 1;
-      } else {
+      if (last >= first) {
         const uint32_t nnz = last - first;
         n = nnz / seg + (nnz % seg != 0u);
       }
@@ -591,6 +611,7 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
   if (tid == 0) {
     ctl->next_unit = 0;
     ctl->invalid = bad;
+    ctl->seg = seg;
     ctl->total_items = carry;
     ctl->total_units = carry * ntiles;
     ctl->ntiles = ntiles;
@@ -605,11 +626,12 @@ enum RowPass { kInit = 0, kFixup = 1 };
 template <int PASS>
 __global__ __launch_bounds__(kBlock) void batch_rows_kernel(
     const float *__restrict__ params, uint32_t samples, uint32_t sp,
-    const MatrixDesc *__restrict__ mats, uint32_t count, uint32_t seg,
-    uint32_t total_chunks, BatchCtl *__restrict__ ctl) {
+    const MatrixDesc *__restrict__ mats, uint32_t count, uint32_t total_chunks,
+    BatchCtl *__restrict__ ctl) {
   WMH_NO_CONTRACT
   const uint32_t lane = lane_id();
   if (PASS == kFixup && uniform(ctl->invalid)) return;
+  const uint32_t seg = uniform(ctl->seg);
   for (uint32_t w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); w < total_chunks;
        w += gridDim.x * kWavesPerBlock) {
     const uint32_t wv = uniform(w);
@@ -756,39 +778,41 @@ hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t tot
 
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
-                             BatchCtl *ctl, const Tuning &tuning, int num_cus,
-                             hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
+                             BatchCtl *ctl, const Tuning &tuning, bool auto_seg,
+                             int num_cus, hipStream_t stream, hipEvent_t ev_begin,
+                             hipEvent_t ev_end) {
   const uint32_t sp = padded_samples(samples);
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
   const uint32_t seg = (uint32_t)tuning.seg;
+  const uint32_t target = auto_seg ? (uint32_t)std::max(1, num_cus * tuning.waves_per_cu * 4) : 0u;
   const uint32_t per_wave = (uint32_t)(kWave * spl);
   const uint32_t ntiles = (samples + per_wave - 1) / per_wave;
   hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(kBlock), 0, stream, mats, count, seg,
-                     ntiles, ctl);
+                     target, ntiles, ctl);
   const unsigned row_grid = grid_for(total_chunks);
   hipLaunchKernelGGL(batch_rows_kernel<kInit>, dim3(row_grid), dim3(kBlock), 0, stream,
-                     params, samples, sp, mats, count, seg, total_chunks, ctl);
+                     params, samples, sp, mats, count, total_chunks, ctl);
   const unsigned grid = (unsigned)std::max(1, num_cus * tuning.waves_per_cu / kWavesPerBlock);
   if (ev_begin) (void)hipEventRecord(ev_begin, stream);
   switch (spl) {
     case 1:
       hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, seg, ctl);
+                         params, dim, samples, sp, mats, count, ctl);
       break;
     case 2:
       hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, seg, ctl);
+                         params, dim, samples, sp, mats, count, ctl);
       break;
     default:
       hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, seg, ctl);
+                         params, dim, samples, sp, mats, count, ctl);
       break;
   }
   if (ev_end) (void)hipEventRecord(ev_end, stream);
   hipLaunchKernelGGL(batch_rows_kernel<kFixup>, dim3(row_grid), dim3(kBlock), 0, stream,
-                     params, samples, sp, mats, count, seg, total_chunks, ctl);
+                     params, samples, sp, mats, count, total_chunks, ctl);
   return hipGetLastError();
 }
 

@@ -162,7 +162,7 @@ struct MinhashCudaGenerator_ {
   uint64_t piece_nnz = 1ull << 28;
   uint64_t piece_rows = 1ull << 20;
   wmh::Tuning tuning{2, 2048, 16};
-  bool seg_auto = true;  // shrink the item size of small batches (host-known nnz)
+  bool seg_auto = true;  // let small batches use smaller work items
   wmh::GraphCache graph;  // hipGraph replay of device-pointer calls (opt-in)
   std::map<uint64_t, wmh::TunedChoice> tuned;  // autotuner cache by batch shape
   wmh::DevBuf tune_buf;
@@ -183,13 +183,10 @@ namespace wmh {
 // Enqueues the chosen hashing kernel over device-resident matrices on `s`,
 // staging persistent-launch descriptors through `ctx`.  The caller owns the
 // synchronization of `ctx` with other streams.
-// `nnz_hint` (0: unknown) is the batch's total nonzero count when the host
-// knows it; it lets small batches use smaller work items.
 // `capturing`: the stream is being captured into a graph, so nothing may
 // allocate or wait on the host (ctx's buffers must already be reserved).
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx, uint64_t nnz_hint,
-                               bool capturing);
+                               int kernel, hipStream_t s, PlanCtx *ctx, bool capturing);
 
 // The hipGraph-replayed form of hash_device_batch for device-pointer calls.
 MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats, int kernel,
