@@ -183,6 +183,29 @@ MHCUDAResult mhcuda_set_timing(MinhashCudaGenerator *gen, int enable);
 MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *gen, float *ms);
 
 /* ------------------------------------------------------------------------ */
+/* Consumers of the hashes (device memory; hash tables are [rows][samples][2]
+ * uint32 as written by the hashing calls of this generator).               */
+
+/* Weighted-Jaccard estimates: out[i] = fraction of samples whose (column, t)
+ * pairs are equal between row pairs[2i] of hashes_a and row pairs[2i+1] of
+ * hashes_b (the estimator datasketch's WeightedMinHash.jaccard computes for
+ * the reference's test.py:104-106).  Samples of rows without a selectable
+ * nonzero (the sentinel pair) never count as equal.  NULL stream: generator
+ * stream and wait; otherwise enqueued on `stream`. */
+MHCUDAResult mhcuda_jaccard_pairs(const MinhashCudaGenerator *gen, const uint32_t *hashes_a,
+                                  const uint32_t *hashes_b, const uint32_t *pairs, uint32_t n,
+                                  float *out, void *stream);
+
+/* LSH band keys for candidate generation: keys[row * bands + g] is a 64-bit
+ * key of the (column, t) pairs of samples [g*r, (g+1)*r), r = samples / bands
+ * (bands must divide samples); rows sharing a key in any band are candidate
+ * pairs.  h = mix(g); for each sample h = mix((h ^ (column << 32 | t)) + g),
+ * with mix the splitmix64 finalizer.  A band holding a sentinel sample gets
+ * the reserved key 2^64 - 1, which consumers skip. */
+MHCUDAResult mhcuda_lsh_band_keys(const MinhashCudaGenerator *gen, const uint32_t *hashes,
+                                  uint32_t rows, uint32_t bands, uint64_t *keys, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Serving loop: continuous batching of CSR requests over one generator.     */
 
 typedef struct MHCUDAServer_ MHCUDAServer;

@@ -22,7 +22,8 @@ EXPORTS = (
     "mhcuda_init", "mhcuda_get_parameters", "mhcuda_retrieve_random_vars",
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
-    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_autotune", "mhcuda_set_graph",
+    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_autotune", "mhcuda_jaccard_pairs",
+    "mhcuda_lsh_band_keys", "mhcuda_set_graph",
     "mhcuda_graph_captures",
     "mhcuda_set_timing", "mhcuda_kernel_time",
     "mhcuda_serve_start", "mhcuda_serve_submit", "mhcuda_serve_wait", "mhcuda_serve_stats",
@@ -101,6 +102,10 @@ def load(path):
                                     ctypes.POINTER(i32)]
     L.mhcuda_autotune.restype = i32
     L.mhcuda_autotune.argtypes = [vp, u32, ctypes.POINTER(Matrix), vp, ctypes.POINTER(TuneResult)]
+    L.mhcuda_jaccard_pairs.restype = i32
+    L.mhcuda_jaccard_pairs.argtypes = [vp, vp, vp, vp, u32, vp, vp]
+    L.mhcuda_lsh_band_keys.restype = i32
+    L.mhcuda_lsh_band_keys.argtypes = [vp, vp, u32, u32, vp, vp]
     L.mhcuda_set_graph.restype = i32
     L.mhcuda_set_graph.argtypes = [vp, i32]
     L.mhcuda_graph_captures.restype = i32

@@ -668,6 +668,35 @@ MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *cgen, float *ms) {
   return mhcudaSuccess;
 }
 
+MHCUDAResult mhcuda_jaccard_pairs(const MinhashCudaGenerator *gen, const uint32_t *hashes_a,
+                                  const uint32_t *hashes_b, const uint32_t *pairs, uint32_t n,
+                                  float *out, void *stream) {
+  if (!gen || !hashes_a || !hashes_b || !pairs || !out) return mhcudaInvalidArguments;
+  if (n == 0) return mhcudaSuccess;
+  int verbosity = gen->verbosity;
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  HIPCH(wmh::launch_jaccard_pairs(hashes_a, hashes_b, pairs, n, gen->samples, out, s),
+        mhcudaRuntimeError);
+  if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_lsh_band_keys(const MinhashCudaGenerator *gen, const uint32_t *hashes,
+                                  uint32_t rows, uint32_t bands, uint64_t *keys, void *stream) {
+  if (!gen || !hashes || !keys || bands == 0 || gen->samples % bands != 0)
+    return mhcudaInvalidArguments;
+  if (rows == 0) return mhcudaSuccess;
+  int verbosity = gen->verbosity;
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  HIPCH(wmh::launch_band_keys(hashes, rows, gen->samples, bands,
+                              reinterpret_cast<unsigned long long *>(keys), s),
+        mhcudaRuntimeError);
+  if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
+  return mhcudaSuccess;
+}
+
 MHCUDAResult mhcuda_set_graph(MinhashCudaGenerator *gen, int enable) {
   if (!gen) return mhcudaInvalidArguments;
   std::lock_guard<std::mutex> lock(gen->mu);

@@ -79,6 +79,16 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
                              int num_cus, hipStream_t stream, hipEvent_t ev_begin = nullptr,
                              hipEvent_t ev_end = nullptr);
 
+// Weighted-Jaccard estimates of row pairs (pairs[2i], pairs[2i+1] index the
+// rows of hash tables a and b, each [rows][samples][2]).
+hipError_t launch_jaccard_pairs(const uint32_t *a, const uint32_t *b, const uint32_t *pairs,
+                                uint32_t n, uint32_t samples, float *out, hipStream_t stream);
+
+// LSH band keys [rows][bands] of a hash table [rows][samples][2]; samples
+// must be a multiple of bands.
+hipError_t launch_band_keys(const uint32_t *hashes, uint32_t rows, uint32_t samples,
+                            uint32_t bands, unsigned long long *keys, hipStream_t stream);
+
 // Row-length histogram of a batch (device MatrixDesc array with chunk_begin
 // filled in): hist[0..32] row counts and hist[33..65] nonzero sums by the bit
 // length of the row's nonzero count.  `hist` must be zeroed beforehand.

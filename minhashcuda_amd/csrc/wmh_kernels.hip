@@ -702,6 +702,69 @@ __global__ __launch_bounds__(kBlock) void batch_histogram_kernel(
     if (local[i]) atomicAdd(&hist[i], local[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Consumers of the hashes: weighted-Jaccard estimates and LSH band keys.
+
+// Estimated weighted Jaccard similarity of row pairs: the fraction of samples
+// whose (column, t) pairs are equal (consistent weighted sampling, Ioffe
+// 2010), counting sentinel samples (rows without a selectable nonzero) as
+// unequal.  One wavefront per pair; lanes compare samples, a ballot counts.
+__global__ __launch_bounds__(kBlock) void jaccard_pairs_kernel(
+    const uint2 *__restrict__ a, const uint2 *__restrict__ b, const uint32_t *__restrict__ pairs,
+    uint32_t n, uint32_t samples, float *__restrict__ out) {
+  const uint32_t lane = lane_id();
+  for (uint32_t w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); w < n;
+       w += gridDim.x * kWavesPerBlock) {
+    const uint32_t p = uniform(w);
+    const uint64_t ra = (uint64_t)uniform(pairs[2 * p]) * samples;
+    const uint64_t rb = (uint64_t)uniform(pairs[2 * p + 1]) * samples;
+    uint32_t equal = 0;
+    for (uint32_t k0 = 0; k0 < samples; k0 += kWave) {
+      bool eq = false;
+      if (k0 + lane < samples) {
+        const uint2 x = a[ra + k0 + lane], y = b[rb + k0 + lane];
+        eq = x.x == y.x && x.y == y.y && !(x.x == WMH_SENTINEL && x.y == WMH_SENTINEL);
+      }
+      equal += (uint32_t)__builtin_popcountll(__ballot(eq));
+    }
+    if (lane == 0) out[p] = (float)equal / (float)samples;
+  }
+}
+
+// 64-bit mixing step (the splitmix64 finalizer constants of Steele, Lea and
+// Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// LSH band keys: band g of a row covers samples [g*r, (g+1)*r), r =
+// samples / bands; its key folds the band's (column, t) pairs in sample
+// order, h = mix64(h ^ (column << 32 | t) + g), starting from h = mix64(g).
+// A band that holds a sentinel sample gets the reserved key 2^64 - 1.  One
+// thread per (row, band).
+__global__ __launch_bounds__(kBlock) void band_keys_kernel(const uint2 *__restrict__ hashes,
+                                                           uint32_t rows, uint32_t samples,
+                                                           uint32_t bands,
+                                                           unsigned long long *__restrict__ keys) {
+  const uint32_t r = samples / bands;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < (uint64_t)rows * bands;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t row = i / bands;
+    const uint32_t g = (uint32_t)(i - row * bands);
+    const uint2 *h = hashes + row * samples + (uint64_t)g * r;
+    uint64_t acc = mix64(g);
+    bool sentinel = false;
+    for (uint32_t k = 0; k < r; k++) {
+      const uint2 x = h[k];
+      sentinel |= x.x == WMH_SENTINEL && x.y == WMH_SENTINEL;
+      acc = mix64((acc ^ (((uint64_t)x.x << 32) | x.y)) + g);
+    }
+    keys[i] = sentinel ? ~0ull : acc;
+  }
+}
+
 unsigned grid_for(uint64_t waves) {
   const uint64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
   return (unsigned)(blocks < (1u << 20) ? (blocks ? blocks : 1) : (1u << 20));
@@ -765,6 +828,25 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
     default:
       return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_jaccard_pairs(const uint32_t *a, const uint32_t *b, const uint32_t *pairs,
+                                uint32_t n, uint32_t samples, float *out, hipStream_t stream) {
+  hipLaunchKernelGGL(jaccard_pairs_kernel, dim3(grid_for(n)), dim3(kBlock), 0, stream,
+                     reinterpret_cast<const uint2 *>(a), reinterpret_cast<const uint2 *>(b),
+                     pairs, n, samples, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_band_keys(const uint32_t *hashes, uint32_t rows, uint32_t samples,
+                            uint32_t bands, unsigned long long *keys, hipStream_t stream) {
+  const uint64_t n = (uint64_t)rows * bands;
+  uint64_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > (1u << 20)) blocks = 1u << 20;
+  hipLaunchKernelGGL(band_keys_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
+                     dim3(kBlock), 0, stream, reinterpret_cast<const uint2 *>(hashes), rows,
+                     samples, bands, keys);
   return hipGetLastError();
 }
 

@@ -227,3 +227,48 @@ def hash_jaccard(h1, h2):
     h1 = np.asarray(h1).reshape(-1, 2)
     h2 = np.asarray(h2).reshape(-1, 2)
     return float(np.mean(np.all(h1 == h2, axis=1)))
+
+
+# --------------------------------------------------------------------------
+# consumers of the hashes (restated for the GPU kernels of the same name)
+
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def mix64(z):
+    """splitmix64 finalizer (Steele, Lea, Flood, OOPSLA 2014) on uint64 arrays."""
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def band_keys(hashes, bands):
+    """LSH band keys [rows][bands] of hashes [rows][samples][2] (uint32):
+    h = mix64(g); h = mix64((h ^ (column << 32 | t)) + g) over the band's
+    samples in order; 2^64 - 1 for a band holding a sentinel sample."""
+    h = np.asarray(hashes, dtype=np.uint32)
+    rows, samples, _ = h.shape
+    r = samples // bands
+    words = (h[..., 0].astype(np.uint64) << np.uint64(32)) | h[..., 1].astype(np.uint64)
+    sentinel = (h[..., 0] == SENTINEL) & (h[..., 1] == SENTINEL)
+    keys = np.empty((rows, bands), np.uint64)
+    with np.errstate(over="ignore"):
+        for g in range(bands):
+            acc = np.full(rows, mix64(np.uint64(g)), np.uint64)
+            for k in range(g * r, (g + 1) * r):
+                acc = mix64((acc ^ words[:, k]) + np.uint64(g))
+            keys[:, g] = np.where(sentinel[:, g * r:(g + 1) * r].any(axis=1),
+                                  np.uint64(0xFFFFFFFFFFFFFFFF), acc)
+    return keys
+
+
+def jaccard_pairs(hashes_a, hashes_b, pairs):
+    """Fraction of samples with equal (column, t) pairs for each row pair
+    (sentinel samples never count as equal)."""
+    a = np.asarray(hashes_a, np.uint32)[np.asarray(pairs)[:, 0]]
+    b = np.asarray(hashes_b, np.uint32)[np.asarray(pairs)[:, 1]]
+    eq = np.all(a == b, axis=2) & ~((a[..., 0] == SENTINEL) & (a[..., 1] == SENTINEL))
+    return (eq.sum(axis=1) / a.shape[1]).astype(np.float32)

@@ -253,3 +253,36 @@ def test_jaccard_estimator():
     err = est - true
     assert abs(err.mean()) < 0.01          # unbiased
     assert np.sqrt(np.mean(err ** 2)) < 0.06  # ~ sqrt(J(1-J)/128)
+
+
+def test_mix64_known_answer():
+    """splitmix64's first output for seed 0 is mix64(0x9e3779b97f4a7c15) =
+    0xe220a8397b1dcdaf (the published reference sequence)."""
+    assert int(oracle.mix64(np.uint64(0x9E3779B97F4A7C15))) == 0xE220A8397B1DCDAF
+    assert int(oracle.mix64(np.uint64(0))) == 0
+
+
+def test_band_keys_and_jaccard_properties():
+    rng = np.random.default_rng(3)
+    h = rng.integers(0, 1000, (6, 16, 2)).astype(np.uint32)
+    h[1] = h[0]
+    h[2, :8] = h[0, :8]                    # first half equal
+    h[3] = oracle.SENTINEL                 # an empty row
+    k = oracle.band_keys(h, 4)
+    assert (k[1] == k[0]).all()
+    assert (k[2, :2] == k[0, :2]).all() and (k[2, 2:] != k[0, 2:]).all()
+    assert (k[3] == np.uint64(0xFFFFFFFFFFFFFFFF)).all()
+    j = oracle.jaccard_pairs(h, h, np.array([[0, 1], [0, 2], [3, 3], [0, 4]]))
+    assert j[0] == 1.0 and j[1] == 0.5 and j[2] == 0.0 and j[3] < 0.2
+
+
+def test_lsh_candidate_pairs():
+    from minhashcuda_amd import lsh
+    keys = np.array([[1, 7], [2, 7], [1, 9], [3, 9], [lsh.RESERVED, 4], [5, 4], [5, 8]],
+                    dtype=np.uint64)
+    pairs = lsh.candidate_pairs(keys)
+    assert pairs.tolist() == [[0, 1], [0, 2], [2, 3], [4, 5], [5, 6]]
+    # oversized buckets are skipped
+    big = np.zeros((100, 1), np.uint64)
+    assert len(lsh.candidate_pairs(big, max_bucket=64)) == 0
+    assert len(lsh.candidate_pairs(big[:10], max_bucket=64)) == 45
