@@ -440,7 +440,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine hashes on a single MI355X "
-              "(multi-GPU row sharding is described in DESIGN.md but not built); "
+              "(multi-GPU row sharding is not built); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np

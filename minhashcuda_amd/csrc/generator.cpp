@@ -114,7 +114,7 @@ int select_device(uint32_t devices, int verbosity) {
     if (chosen < 0) {
       chosen = dev;
     } else {
-      INFO("device %d also selected: this build hashes on one GPU (#%d)\n", dev, chosen);
+      INFO("device %d also selected: the engine hashes on one GPU (#%d)\n", dev, chosen);
     }
   }
   return chosen;
