@@ -183,6 +183,35 @@ MHCUDAResult mhcuda_set_timing(MinhashCudaGenerator *gen, int enable);
 MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *gen, float *ms);
 
 /* ------------------------------------------------------------------------ */
+/* Parameter files: the three tables (or, for tables generated on the device,
+ * just the seed) in a versioned, checksummed file, so that a later process
+ * reproduces the same hashes.  They extend the retrieve / assign pair
+ * (reference minhashcuda.cc:254-285) and the deferred-init flow of the
+ * reference's test.py:154-182 to a file.  Format version 1: a 64-byte header
+ * (magic "WMHPARAM", version, kind, dim, samples, seed, generator recipe,
+ * three table checksums, header checksum), then for kind 1 the tables rs,
+ * ln_cs, betas, each [samples][dim] float32 little-endian.
+ * File errors return mhcudaMemoryCopyError; a malformed file, a checksum
+ * mismatch or a file for another dim / sample count returns
+ * mhcudaInvalidArguments. */
+
+/* Writes the generator's tables to `path` (through a temporary file renamed
+ * into place).  With allow_seed_only != 0 and tables generated on the device
+ * from a seed, only the seed is written (kind 2, 64 bytes).  A deferred
+ * generator whose tables were never assigned is rejected. */
+MHCUDAResult mhcuda_save_random_vars(const MinhashCudaGenerator *gen, const char *path,
+                                     int allow_seed_only);
+
+/* Loads a parameter file into a generator of the same dim and sample count
+ * (typically created with deferred = 1).  If the tables fail their checksum
+ * the generator's tables are zeroed, as after a deferred init, and
+ * mhcudaInvalidArguments is returned. */
+MHCUDAResult mhcuda_load_random_vars(const MinhashCudaGenerator *gen, const char *path);
+
+/* Reads dim and samples from a parameter file's header (no device work). */
+MHCUDAResult mhcuda_params_file_info(const char *path, uint32_t *dim, uint16_t *samples);
+
+/* ------------------------------------------------------------------------ */
 /* Consumers of the hashes (device memory; hash tables are [rows][samples][2]
  * uint32 as written by the hashing calls of this generator).               */
 

@@ -39,6 +39,7 @@ SOURCES = [
     ("generator.cpp", False),
     ("serve.cpp", False),
     ("autotune.cpp", False),
+    ("params_file.cpp", False),
     ("python_module.cpp", False),
 ]
 

@@ -23,7 +23,8 @@ EXPORTS = (
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
     "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_autotune", "mhcuda_jaccard_pairs",
-    "mhcuda_lsh_band_keys", "mhcuda_set_graph",
+    "mhcuda_lsh_band_keys", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
+    "mhcuda_params_file_info", "mhcuda_set_graph",
     "mhcuda_graph_captures",
     "mhcuda_set_timing", "mhcuda_kernel_time",
     "mhcuda_serve_start", "mhcuda_serve_submit", "mhcuda_serve_wait", "mhcuda_serve_stats",
@@ -106,6 +107,13 @@ def load(path):
     L.mhcuda_jaccard_pairs.argtypes = [vp, vp, vp, vp, u32, vp, vp]
     L.mhcuda_lsh_band_keys.restype = i32
     L.mhcuda_lsh_band_keys.argtypes = [vp, vp, u32, u32, vp, vp]
+    L.mhcuda_save_random_vars.restype = i32
+    L.mhcuda_save_random_vars.argtypes = [vp, ctypes.c_char_p, i32]
+    L.mhcuda_load_random_vars.restype = i32
+    L.mhcuda_load_random_vars.argtypes = [vp, ctypes.c_char_p]
+    L.mhcuda_params_file_info.restype = i32
+    L.mhcuda_params_file_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(u32),
+                                          ctypes.POINTER(ctypes.c_uint16)]
     L.mhcuda_set_graph.restype = i32
     L.mhcuda_set_graph.argtypes = [vp, i32]
     L.mhcuda_graph_captures.restype = i32

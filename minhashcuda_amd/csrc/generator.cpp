@@ -359,6 +359,8 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
   } else {
     if (wmh::launch_gen_params(gen->params, dim, samples, seed, gen->stream) != hipSuccess)
       FAIL(mhcudaRuntimeError);
+    gen->params_source = MinhashCudaGenerator::kParamsSeeded;
+    gen->seed = seed;
   }
   if (hipStreamSynchronize(gen->stream) != hipSuccess) FAIL(mhcudaRuntimeError);
   if (verbosity > 1 && print_memory_stats(dev) != mhcudaSuccess) FAIL(mhcudaRuntimeError);
@@ -414,6 +416,7 @@ MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *cgen, const f
   HIPCH(gen->staging.reserve((size_t)chunk * gen->samples * sizeof(float)),
         mhcudaMemoryAllocationFailure);
   const float *src[3] = {rs, betas, ln_cs};
+  gen->params_source = MinhashCudaGenerator::kParamsNone;  // until every table is in
   for (uint32_t which = 0; which < 3; which++) {
     for (uint32_t j0 = 0; j0 < gen->dim; j0 += chunk) {
       const uint32_t n = std::min(chunk, gen->dim - j0);
@@ -428,6 +431,7 @@ MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *cgen, const f
       HIPCH(hipStreamSynchronize(gen->stream), mhcudaMemoryCopyError);
     }
   }
+  gen->params_source = MinhashCudaGenerator::kParamsTables;
   return mhcudaSuccess;
 }
 

@@ -7,9 +7,13 @@
 //   minhash_cuda_assign_vars(gen, rs, ln_cs, betas)      python.cc:197-272
 //   minhash_cuda_calc(gen, matrix, row_start=0,
 //                     row_finish=0xffffffff)             python.cc:274-399
+//     (matrices of 2^32 or more nonzeros, which the reference's uint32
+//     force-cast of indptr would corrupt, are hashed in row slices)
 //   minhash_cuda_fini(gen)                               python.cc:401-410
 // plus minhash_cuda_calc_batch(gen, [matrices], kernel=0), which hashes
-// several csr_matrix objects in one batched launch.
+// several csr_matrix objects in one batched launch, and
+// minhash_cuda_save_vars(gen, path, seed_only=False) /
+// minhash_cuda_load_vars(gen, path) for parameter files.
 //
 // Exception mapping (python.cc:119-142): InvalidArguments and NoSuchDevice ->
 // ValueError, MemoryAllocationFailure -> MemoryError, MemoryCopyError ->
@@ -21,6 +25,7 @@
 #include <numpy/arrayobject.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <ctime>
 #include <vector>
 
@@ -107,7 +112,8 @@ Ref as_u32(PyObject *obj) {
   return Ref(PyArray_FROM_OTF(obj, NPY_UINT32, NPY_ARRAY_FORCECAST | NPY_ARRAY_IN_ARRAY));
 }
 
-bool load_csr(PyObject *matrix, Csr *out) {
+// python.cc:281-294: only scipy.sparse.csr_matrix instances are accepted.
+bool check_csr_type(PyObject *matrix) {
   Ref sparse(PyImport_ImportModule("scipy.sparse"));
   if (!sparse) {
     PyErr_SetString(PyExc_ImportError, "Failed to import scipy.sparse.csr_matrix");
@@ -122,9 +128,14 @@ bool load_csr(PyObject *matrix, Csr *out) {
     PyErr_SetString(PyExc_TypeError, "The second argument must be of type scipy.sparse.csr_matrix");
     return false;
   }
+  return true;
+}
+
+// data (float32, no FORCECAST: float64 is refused, as in python.cc:311) and
+// indices as uint32 bit patterns.
+bool load_values(PyObject *matrix, Csr *out) {
   Ref data(PyObject_GetAttrString(matrix, "data"));
   if (!data) return false;
-  // float32 without FORCECAST: float64 data is refused, as in python.cc:311
   out->weights = Ref(PyArray_FROM_OTF(data.get(), NPY_FLOAT32, NPY_ARRAY_IN_ARRAY));
   if (!out->weights) {
     PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.data to numpy.array");
@@ -137,17 +148,30 @@ bool load_csr(PyObject *matrix, Csr *out) {
     PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.indices to numpy.array");
     return false;
   }
+  return true;
+}
+
+// The last indptr entry (the matrix's nonzero count when indptr[0] == 0).
+bool last_position(PyObject *matrix, long long *last) {
   Ref indptr(PyObject_GetAttrString(matrix, "indptr"));
   if (!indptr) return false;
-  // the engine's indptr is uint32 (minhashcuda.h:117): larger matrices must be
-  // hashed in row slices
   Ref last_obj(PySequence_GetItem(indptr.get(), -1));
   if (!last_obj) return false;
-  const long long last = PyLong_AsLongLong(last_obj.get());
-  if (last == -1 && PyErr_Occurred()) return false;
+  *last = PyLong_AsLongLong(last_obj.get());
+  return !(*last == -1 && PyErr_Occurred());
+}
+
+bool load_csr(PyObject *matrix, Csr *out) {
+  if (!check_csr_type(matrix) || !load_values(matrix, out)) return false;
+  Ref indptr(PyObject_GetAttrString(matrix, "indptr"));
+  if (!indptr) return false;
+  // the engine's indptr is uint32 (minhashcuda.h:117); minhash_cuda_calc
+  // cuts larger matrices into row slices itself, a batch must be given them
+  long long last = 0;
+  if (!last_position(matrix, &last)) return false;
   if (last > 0xffffffffLL) {
     PyErr_SetString(PyExc_ValueError,
-                    "csr_matrix has 2^32 or more nonzeros: hash it in row slices");
+                    "csr_matrix has 2^32 or more nonzeros: hash it with minhash_cuda_calc");
     return false;
   }
   out->rows = as_u32(indptr.get());
@@ -240,6 +264,60 @@ PyObject *py_assign_vars(PyObject *, PyObject *args) {
   Py_RETURN_NONE;
 }
 
+// Nonzeros per row slice of a matrix beyond the engine's uint32 indptr
+// (MHCUDA_PY_SLICE_NNZ overrides it).
+uint64_t slice_nnz() {
+  if (const char *env = std::getenv("MHCUDA_PY_SLICE_NNZ")) {
+    const unsigned long long v = std::strtoull(env, nullptr, 10);
+    if (v > 0 && v <= 0xffffffffull) return v;
+  }
+  return 1ull << 31;
+}
+
+// Rows [row_start, row_finish) of a matrix whose selected nonzeros exceed
+// slice_nnz() (in particular 2^32 or more, which the engine's uint32 indptr
+// cannot address): consecutive row slices of at most slice_nnz() nonzeros,
+// each with its own rebased uint32 indptr, with weights and indices passed in
+// place at the slice's offset, all hashed in one pipelined mhcuda_calc_batch
+// call.  The hashes are those of a single call over the same rows.
+PyObject *calc_sliced(MinhashCudaGenerator *gen, uint32_t row_start, uint32_t row_finish,
+                      const Csr &csr, PyArrayObject *indptr) {
+  const int64_t *ip = static_cast<const int64_t *>(PyArray_DATA(indptr));
+  const uint64_t limit = slice_nnz();
+  std::vector<std::vector<uint32_t>> rebased;
+  std::vector<MHCUDAMatrix> mats;
+  const MinhashCudaGeneratorParameters p = mhcuda_get_parameters(gen);
+  Ref out(new_hashes(row_finish - row_start, p.samples));
+  if (!out) return nullptr;
+  uint32_t *output = static_cast<uint32_t *>(PyArray_DATA(out.arr()));
+  const float *weights = static_cast<const float *>(PyArray_DATA(csr.weights.arr()));
+  const uint32_t *cols = static_cast<const uint32_t *>(PyArray_DATA(csr.cols.arr()));
+  for (uint32_t r = row_start; r < row_finish;) {
+    const int64_t base = ip[r];
+    uint32_t e = r + 1;
+    while (e < row_finish && (uint64_t)(ip[e + 1] - base) <= limit) e++;
+    if ((uint64_t)(ip[e] - base) > 0xffffffffull) {
+      PyErr_Format(PyExc_ValueError, "row %u has 2^32 or more nonzeros", r);
+      return nullptr;
+    }
+    std::vector<uint32_t> rows(e - r + 1);
+    for (uint32_t i = r; i <= e; i++) rows[i - r] = (uint32_t)(ip[i] - base);
+    rebased.push_back(std::move(rows));
+    mats.push_back(MHCUDAMatrix{weights + base, cols + base, rebased.back().data(), e - r,
+                                output + (uint64_t)(r - row_start) * p.samples * 2u});
+    r = e;
+  }
+  // the vectors moved into `rebased` keep their buffers: fix the pointers up
+  for (size_t i = 0; i < mats.size(); i++) mats[i].rows = rebased[i].data();
+  MHCUDAResult result;
+  Py_BEGIN_ALLOW_THREADS
+  result = mhcuda_calc_batch(gen, static_cast<uint32_t>(mats.size()), mats.data(),
+                             mhcudaKernelAuto);
+  Py_END_ALLOW_THREADS
+  if (result != mhcudaSuccess) return raise(result, "minhash_cuda_calc");
+  return out.release();
+}
+
 PyObject *py_calc(PyObject *, PyObject *args, PyObject *kwargs) {
   unsigned long long ptr = 0;
   PyObject *matrix = nullptr;
@@ -250,11 +328,28 @@ PyObject *py_calc(PyObject *, PyObject *args, PyObject *kwargs) {
     return nullptr;
   MinhashCudaGenerator *gen = gen_from(ptr);
   if (!gen) return nullptr;
+  if (!check_csr_type(matrix)) return nullptr;
   Csr csr;
-  if (!load_csr(matrix, &csr)) return nullptr;
-  uint32_t length = csr.length;
-  // row slicing, python.cc:335-359; the indptr stays absolute, so the slice
-  // is zero-copy (no rebased indptr is built)
+  if (!load_values(matrix, &csr)) return nullptr;
+  Ref indptr_obj(PyObject_GetAttrString(matrix, "indptr"));
+  if (!indptr_obj) return nullptr;
+  // positions as int64 (a copy of rows + 1 values unless already int64)
+  Ref indptr(PyArray_FROM_OTF(indptr_obj.get(), NPY_INT64, NPY_ARRAY_IN_ARRAY | NPY_ARRAY_FORCECAST));
+  if (!indptr) {
+    PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.indptr to numpy.array");
+    return nullptr;
+  }
+  const npy_intp n = PyArray_SIZE(indptr.arr());
+  if (n < 1) {
+    PyErr_SetString(PyExc_ValueError, "csr_matrix.indptr is empty");
+    return nullptr;
+  }
+  if (n - 1 > 0xffffffffLL) {
+    PyErr_SetString(PyExc_ValueError, "csr_matrix has 2^32 or more rows");
+    return nullptr;
+  }
+  uint32_t length = static_cast<uint32_t>(n - 1);
+  // row slicing, python.cc:335-359
   if (row_start >= length) {
     PyErr_SetString(PyExc_ValueError,
                     "minhash_cuda_calc: row_start must be less than the total number of rows "
@@ -264,6 +359,25 @@ PyObject *py_calc(PyObject *, PyObject *args, PyObject *kwargs) {
   row_finish = std::min<uint32_t>(row_finish, length);
   if (row_finish <= row_start) {
     PyErr_SetString(PyExc_ValueError, "minhash_cuda_calc: row_finish must be greater than row_start.");
+    return nullptr;
+  }
+  // the selected rows must address positions inside data and indices
+  const int64_t *ip = static_cast<const int64_t *>(PyArray_DATA(indptr.arr()));
+  const int64_t avail = std::min<int64_t>(PyArray_SIZE(csr.weights.arr()), PyArray_SIZE(csr.cols.arr()));
+  for (uint32_t r = row_start; r < row_finish; r++) {
+    if (ip[r] < 0 || ip[r + 1] < ip[r] || ip[r + 1] > avail) {
+      PyErr_SetString(PyExc_ValueError,
+                      "csr_matrix.indptr must be non-decreasing and within data and indices");
+      return nullptr;
+    }
+  }
+  if (ip[length] > 0xffffffffLL || (uint64_t)(ip[row_finish] - ip[row_start]) > slice_nnz())
+    return calc_sliced(gen, row_start, row_finish, csr, indptr.arr());
+  // the whole matrix fits the uint32 indptr: it stays absolute, so the row
+  // slice is zero-copy (no rebased indptr is built)
+  csr.rows = as_u32(indptr_obj.get());
+  if (!csr.rows) {
+    PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.indptr to numpy.array");
     return nullptr;
   }
   length = row_finish - row_start;
@@ -320,6 +434,61 @@ PyObject *py_calc_batch(PyObject *, PyObject *args, PyObject *kwargs) {
   return outs.release();
 }
 
+// Parameter files (see mhcuda_save_random_vars): ValueError for a malformed or
+// mismatched file, OSError when the file cannot be read or written.
+PyObject *raise_file(MHCUDAResult code, const char *fn, const char *path) {
+  if (code == mhcudaInvalidArguments) {
+    PyErr_Format(PyExc_ValueError,
+                 "%s: %s is not a valid parameter file for this generator "
+                 "(or the generator holds no tables)", fn, path);
+    return nullptr;
+  }
+  if (code == mhcudaMemoryCopyError) {
+    PyErr_Format(PyExc_OSError, "%s: failed to read or write %s", fn, path);
+    return nullptr;
+  }
+  return raise(code, fn);
+}
+
+PyObject *py_save_vars(PyObject *, PyObject *args, PyObject *kwargs) {
+  unsigned long long ptr = 0;
+  PyObject *path_obj = nullptr;
+  int seed_only = 0;
+  static const char *kwlist[] = {"gen", "path", "seed_only", nullptr};
+  if (!PyArg_ParseTupleAndKeywords(args, kwargs, "KO&|p", const_cast<char **>(kwlist), &ptr,
+                                   PyUnicode_FSConverter, &path_obj, &seed_only))
+    return nullptr;
+  Ref path(path_obj);
+  MinhashCudaGenerator *gen = gen_from(ptr);
+  if (!gen) return nullptr;
+  const char *p = PyBytes_AS_STRING(path.get());
+  MHCUDAResult result;
+  Py_BEGIN_ALLOW_THREADS
+  result = mhcuda_save_random_vars(gen, p, seed_only);
+  Py_END_ALLOW_THREADS
+  if (result != mhcudaSuccess) return raise_file(result, "minhash_cuda_save_vars", p);
+  Py_RETURN_NONE;
+}
+
+PyObject *py_load_vars(PyObject *, PyObject *args, PyObject *kwargs) {
+  unsigned long long ptr = 0;
+  PyObject *path_obj = nullptr;
+  static const char *kwlist[] = {"gen", "path", nullptr};
+  if (!PyArg_ParseTupleAndKeywords(args, kwargs, "KO&", const_cast<char **>(kwlist), &ptr,
+                                   PyUnicode_FSConverter, &path_obj))
+    return nullptr;
+  Ref path(path_obj);
+  MinhashCudaGenerator *gen = gen_from(ptr);
+  if (!gen) return nullptr;
+  const char *p = PyBytes_AS_STRING(path.get());
+  MHCUDAResult result;
+  Py_BEGIN_ALLOW_THREADS
+  result = mhcuda_load_random_vars(gen, p);
+  Py_END_ALLOW_THREADS
+  if (result != mhcudaSuccess) return raise_file(result, "minhash_cuda_load_vars", p);
+  Py_RETURN_NONE;
+}
+
 PyObject *py_fini(PyObject *, PyObject *args) {
   unsigned long long ptr = 0;
   if (!PyArg_ParseTuple(args, "K", &ptr)) return nullptr;
@@ -343,6 +512,15 @@ PyMethodDef methods[] = {
      reinterpret_cast<PyCFunction>(reinterpret_cast<void (*)(void)>(py_calc_batch)),
      METH_VARARGS | METH_KEYWORDS,
      "Calculates Weighted MinHashes of several csr_matrix objects in one batch."},
+    {"minhash_cuda_save_vars",
+     reinterpret_cast<PyCFunction>(reinterpret_cast<void (*)(void)>(py_save_vars)),
+     METH_VARARGS | METH_KEYWORDS,
+     "Writes the generator's random variables to a parameter file (seed_only: just the "
+     "seed when the tables were generated on the device)."},
+    {"minhash_cuda_load_vars",
+     reinterpret_cast<PyCFunction>(reinterpret_cast<void (*)(void)>(py_load_vars)),
+     METH_VARARGS | METH_KEYWORDS,
+     "Loads a parameter file into a generator of the same dim and samples."},
     {"minhash_cuda_fini", py_fini, METH_VARARGS,
      "Disposes Weighted MinHash internal state on the GPU."},
     {nullptr, nullptr, 0, nullptr}};

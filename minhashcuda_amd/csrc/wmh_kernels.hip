@@ -233,6 +233,7 @@ __device__ __forceinline__ void hash_range(const float *__restrict__ params,
                                            const uint32_t *__restrict__ cols,
                                            uint32_t begin, uint32_t end,
                                            Argmin<V> &acc) {
+  static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
   const uint32_t lane = lane_id();
   const char *pbase = reinterpret_cast<const char *>(params);
   for (uint32_t base = begin; base < end; base += kWave) {
@@ -243,8 +244,10 @@ __device__ __forceinline__ void hash_range(const float *__restrict__ params,
       c = cols[base + lane];
       lw = wmh_logf(weights[base + lane]);  // once per nonzero (kernel.cu:95)
     }
-    uint32_t i = 0;
-    for (; i + U <= n; i += U) {
+    // The last group of a chunk may run past n: lanes >= n hold the column
+    // WMH_SENTINEL (>= dim), so those slots read column 0's block and are
+    // never selected, and the group keeps all of its gathers in flight.
+    for (uint32_t i = 0; i < n; i += U) {
       uint32_t j[U];
       float lwu[U];
       float r[U][V], beta[U][V], lc[U][V];
@@ -259,16 +262,6 @@ __device__ __forceinline__ void hash_range(const float *__restrict__ params,
       }
 #pragma unroll
       for (int u = 0; u < U; u++) fold<V>(r[u], beta[u], lc[u], j[u], lwu[u], j[u] < dim, acc);
-    }
-    for (; i < n; i++) {
-      const uint32_t j = readlane_u32(c, i);
-      const float lwi = readlane_f32(lw, i);
-      const char *blk = pbase + (uint64_t)(j < dim ? j : 0u) * block_bytes;
-      float r[V], beta[V], lc[V];
-      load_v<V>(at(blk, lo.r), r);
-      load_v<V>(at(blk, lo.beta), beta);
-      load_v<V>(at(blk, lo.lc), lc);
-      fold<V>(r, beta, lc, j, lwi, j < dim, acc);
     }
   }
 }
@@ -418,6 +411,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
                                           uint32_t sp, uint32_t samples, uint32_t k0,
                                           LaneOffsets lo, const MatrixDesc &M,
                                           uint32_t a, uint32_t b) {
+  static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
   const uint32_t lane = lane_id();
   const char *pbase = reinterpret_cast<const char *>(params);
   const uint64_t block_bytes = 12ull * sp;
@@ -460,8 +454,10 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
       c = cols[cb + lane];
       lw = wmh_logf(weights[cb + lane]);
     }
-    uint32_t i = 0;
-    for (; i + U <= n; i += U) {
+    // As in hash_range, the last group may run past n (slots >= n hold the
+    // column WMH_SENTINEL and are never selected); only real positions can
+    // end a row.
+    for (uint32_t i = 0; i < n; i += U) {
       uint32_t j[U];
       float lwu[U];
       float rr[U][V], beta[U][V], lc[U][V];
@@ -478,25 +474,10 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
       for (int u = 0; u < U; u++) {
         const uint32_t p = cb + i + u;
         fold<V>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u], j[u] < dim, acc);
-        if (p + 1 == row_end) {
+        if (i + u < n && p + 1 == row_end) {
           flush_row<V>(M.output, r, samples, k0, split, acc);
           if (p + 1 < b) next_row();
         }
-      }
-    }
-    for (; i < n; i++) {
-      const uint32_t j = readlane_u32(c, i);
-      const float lwi = readlane_f32(lw, i);
-      const char *blk = pbase + (uint64_t)(j < dim ? j : 0u) * block_bytes;
-      float rr[V], beta[V], lc[V];
-      load_v<V>(at(blk, lo.r), rr);
-      load_v<V>(at(blk, lo.beta), beta);
-      load_v<V>(at(blk, lo.lc), lc);
-      const uint32_t p = cb + i;
-      fold<V>(rr, beta, lc, split ? p : j, lwi, j < dim, acc);
-      if (p + 1 == row_end) {
-        flush_row<V>(M.output, r, samples, k0, split, acc);
-        if (p + 1 < b) next_row();
       }
     }
   }

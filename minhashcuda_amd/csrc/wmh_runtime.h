@@ -150,6 +150,11 @@ struct MinhashCudaGenerator_ {
   uint32_t sp = 0;
   int num_cus = 0;
   float *params = nullptr;            // [dim][3][sp]
+  // where the tables came from: nothing yet (deferred init), the on-device
+  // generator with `seed`, or host tables (assign / a parameter file)
+  enum { kParamsNone = 0, kParamsSeeded = 1, kParamsTables = 2 };
+  int params_source = kParamsNone;
+  uint32_t seed = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;
   // grow-only caches, reference minhashcuda.cc:27-31
