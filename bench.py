@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--spl", type=int, default=0, help="samples per lane (0: default)")
     p.add_argument("--seg", type=int, default=0, help="nonzeros per work item (0: default)")
     p.add_argument("--wpc", type=int, default=0, help="persistent waves per CU (0: default)")
+    p.add_argument("--stripe-mib", type=int, default=-1,
+                   help="persistent column-stripe size in MiB (0: off, -1: library default)")
     p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
     p.add_argument("--no-graph", action="store_true",
                    help="launch eagerly instead of replaying a captured hipGraph")
@@ -121,6 +123,8 @@ class Engine:
         self.gen = libMHCUDA.minhash_cuda_init(dim, SAMPLES, seed=seed, devices=1)
         self.g = ctypes.c_void_p(self.gen)
         capi_mod.check(self.capi.mhcuda_set_tuning(self.g, args.spl, args.seg, args.wpc), "tuning")
+        if getattr(args, "stripe_mib", -1) >= 0:
+            capi_mod.check(self.capi.mhcuda_set_stripes(self.g, args.stripe_mib), "stripes")
         capi_mod.check(self.capi.mhcuda_set_timing(self.g, 1), "timing")
         capi_mod.check(self.capi.mhcuda_set_graph(self.g, 0 if args.no_graph else 1), "graph")
         self.graph = not args.no_graph
@@ -128,7 +132,10 @@ class Engine:
     def tuning(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self.capi.mhcuda_get_tuning(self.g, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        mib, k, n = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_uint32()
+        self.capi.mhcuda_get_stripes(self.g, ctypes.byref(mib), ctypes.byref(k), ctypes.byref(n))
         return {"samples_per_lane": a.value, "segment": b.value, "waves_per_cu": c.value,
+                "stripe_mib": mib.value, "stripes": k.value, "striped_row_min_nnz": n.value,
                 "hip_graph": self.graph}
 
     def kernel_ms(self):
@@ -144,11 +151,12 @@ class Engine:
         self.L.minhash_cuda_fini(self.gen)
 
 
-def measured_traffic(workload, kernel, alg_bytes):
+def measured_traffic(workload, kernel, alg_bytes, stripe_mib=None):
     """HBM bytes per launch of `kernel` on this workload as measured by the
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
     profiles/hbm_traffic.json (scaled by a copy-kernel calibration); None when
-    no measurement of this exact configuration is recorded."""
+    no measurement of this exact configuration (input size, kernel and, for
+    the persistent kernel, column-stripe size) is recorded."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
@@ -156,6 +164,8 @@ def measured_traffic(workload, kernel, alg_bytes):
     except (OSError, KeyError, ValueError):
         return None
     if rec.get("kernel") != kernel or rec.get("algorithmic_bytes") != alg_bytes:
+        return None
+    if stripe_mib is not None and rec.get("stripe_mib") != stripe_mib:
         return None
     return rec["traffic_bytes"]
 
@@ -417,7 +427,8 @@ def large_bench(args, torch):
                         "chunk_rows": chunk_rows, "chunk_nnz": nnz}, **engine.tuning()),
         "roofline": {"bound": "hbm", "kernel": "wmh_persistent_kernel", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("large", "wmh_persistent_kernel", alg),
+                     "traffic": measured_traffic("large", "wmh_persistent_kernel", alg,
+                                                 engine.tuning()["stripe_mib"]),
                      "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms},
         "end_to_end": {"rows": total_rows, "nnz": e2e_nnz,
                        "seconds": min(ends), "value": e2e_units / min(ends),
@@ -540,7 +551,9 @@ def main():
                        **engine.tuning()),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, kname, alg),
+                     "traffic": measured_traffic(args.workload, kname, alg,
+                                                 engine.tuning()["stripe_mib"]
+                                                 if kname == "wmh_persistent_kernel" else None),
                      "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms,
                      "timed_span": ("hipGraph replay: descriptor upload, plan, init, %s, "
                                     "fix-up (HIP events around the graph launch)" % kname

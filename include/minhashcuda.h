@@ -139,6 +139,18 @@ MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen,
                                int *samples_per_lane, int *segment,
                                int *waves_per_cu);
 
+/* Column stripes of the persistent kernel: with a parameter table larger
+ * than `stripe_mib` MiB, rows of many nonzeros are cut into as many pieces as
+ * the table has stripes of that size, and all first pieces are hashed before
+ * all second pieces, and so on, so that concurrent gathers share one stripe
+ * of the table (for rows with sorted, spread columns).  0 turns stripes off.
+ * The hashes do not depend on it. */
+MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib);
+/* The stripe size, the resulting stripe count for this generator's table
+ * (1: off) and the smallest striped row (nonzeros; 0 when off). */
+MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib,
+                                uint32_t *stripes, uint32_t *min_nnz);
+
 /* Result of mhcuda_autotune. */
 typedef struct {
   uint64_t key;              /* samples | p50 bin << 16 | p90 bin << 24 | size class << 32 */
@@ -151,13 +163,15 @@ typedef struct {
   float sample_ms;           /* time of the chosen configuration on the timed prefix */
   uint64_t sample_nnz;       /* nonzeros of the timed prefix */
   int cached;                /* 1: taken from the cache, nothing was timed */
+  int stripe_mib;            /* column-stripe size (0: no stripes) */
 } MHCUDATuneResult;
 
 /* Autotunes the kernel choice and tuning for a batch of device-resident
  * matrices: measures the batch's nonzero-weighted row-length histogram on the
  * device, and unless a batch of the same shape (histogram bins of the median
  * and 90th-percentile nonzero, size class, samples) was tuned before, times
- * candidates on a prefix of up to 2^28 nonzeros.  The choice becomes the
+ * candidates on a prefix of up to 2^28 nonzeros (kernel, samples per lane,
+ * item size, waves per CU, column-stripe size).  The choice becomes the
  * generator's tuning (kernel mhcudaKernelAuto then uses it).  The outputs of
  * the timed prefix are overwritten. */
 MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t count,

@@ -22,7 +22,8 @@ EXPORTS = (
     "mhcuda_init", "mhcuda_get_parameters", "mhcuda_retrieve_random_vars",
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
-    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_autotune", "mhcuda_jaccard_pairs",
+    "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_stripes", "mhcuda_get_stripes",
+    "mhcuda_autotune", "mhcuda_jaccard_pairs",
     "mhcuda_lsh_band_keys", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
     "mhcuda_params_file_info", "mhcuda_set_graph",
     "mhcuda_graph_captures",
@@ -51,7 +52,8 @@ class TuneResult(ctypes.Structure):
                 ("histogram_nnz", ctypes.c_uint64), ("kernel", ctypes.c_int),
                 ("samples_per_lane", ctypes.c_int), ("segment", ctypes.c_int),
                 ("waves_per_cu", ctypes.c_int), ("sample_ms", ctypes.c_float),
-                ("sample_nnz", ctypes.c_uint64), ("cached", ctypes.c_int)]
+                ("sample_nnz", ctypes.c_uint64), ("cached", ctypes.c_int),
+                ("stripe_mib", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -101,6 +103,11 @@ def load(path):
     L.mhcuda_get_tuning.restype = i32
     L.mhcuda_get_tuning.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                     ctypes.POINTER(i32)]
+    L.mhcuda_set_stripes.restype = i32
+    L.mhcuda_set_stripes.argtypes = [vp, i32]
+    L.mhcuda_get_stripes.restype = i32
+    L.mhcuda_get_stripes.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(u32),
+                                     ctypes.POINTER(u32)]
     L.mhcuda_autotune.restype = i32
     L.mhcuda_autotune.argtypes = [vp, u32, ctypes.POINTER(Matrix), vp, ctypes.POINTER(TuneResult)]
     L.mhcuda_jaccard_pairs.restype = i32

@@ -82,6 +82,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
     res.samples_per_lane = gen->tuning.spl;
     res.segment = gen->tuning.seg;
     res.waves_per_cu = gen->tuning.waves_per_cu;
+    res.stripe_mib = gen->tuning.stripe_mib;
     res.cached = 1;
     if (result) *result = res;
     return mhcudaSuccess;
@@ -178,6 +179,17 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
         c.tuning.waves_per_cu = wpc;
         consider(c);
       }
+      // column stripes (only where the table spans several stripes)
+      const Candidate after_wpc = best;
+      uint32_t k = 1, min_nnz = 0;
+      for (int mib : {0, 16, 32, 64, 256}) {
+        if (mib == after_wpc.tuning.stripe_mib) continue;
+        wmh::stripe_shape(gen->dim, gen->samples, mib, &k, &min_nnz);
+        if (mib != 0 && k < 2) continue;
+        Candidate c = after_wpc;
+        c.tuning.stripe_mib = mib;
+        consider(c);
+      }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -185,9 +197,11 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
     gen->tuning = saved;
     if (r != mhcudaSuccess) return r;
     if (verbosity > 0)
-      printf("autotune key %llx: kernel %d spl %d seg %d waves/CU %d (%.3f ms on %llu nnz)\n",
+      printf("autotune key %llx: kernel %d spl %d seg %d waves/CU %d stripes %d MiB "
+             "(%.3f ms on %llu nnz)\n",
              (unsigned long long)key, best.kernel, best.tuning.spl, best.tuning.seg,
-             best.tuning.waves_per_cu, best_ms, (unsigned long long)have);
+             best.tuning.waves_per_cu, best.tuning.stripe_mib, best_ms,
+             (unsigned long long)have);
     wmh::TunedChoice choice;
     choice.kernel = best.kernel;
     choice.tuning = best.tuning;
@@ -203,6 +217,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
   res.samples_per_lane = it->second.tuning.spl;
   res.segment = it->second.tuning.seg;
   res.waves_per_cu = it->second.tuning.waves_per_cu;
+  res.stripe_mib = it->second.tuning.stripe_mib;
   res.sample_ms = it->second.ms;
   res.sample_nnz = it->second.sample_nnz;
   if (result) *result = res;

@@ -143,6 +143,27 @@ namespace wmh {
 
 // Enqueues the persistent batched path on `s`: descriptors go through the
 // pinned staging buffer into device memory next to the control block.
+// Device plan buffer of a persistent launch: descriptors, control block and,
+// with column stripes, the striped-row list (one slot per row of the batch).
+struct PlanLayout {
+  size_t ctl_off, list_off, bytes;
+};
+
+static PlanLayout plan_layout(const MinhashCudaGenerator *gen,
+                              const std::vector<MatrixDesc> &mats) {
+  PlanLayout l;
+  const size_t desc_bytes = sizeof(MatrixDesc) * mats.size();
+  l.ctl_off = (desc_bytes + 255) & ~(size_t)255;
+  l.list_off = (l.ctl_off + sizeof(BatchCtl) + 255) & ~(size_t)255;
+  uint32_t stripes = 1, min_nnz = 0;
+  stripe_shape(gen->dim, gen->samples, gen->tuning.stripe_mib, &stripes, &min_nnz);
+  uint64_t rows = 0;
+  if (stripes > 1)
+    for (const MatrixDesc &m : mats) rows += m.length;
+  l.bytes = l.list_off + sizeof(StripedRow) * std::max<uint64_t>(rows, 1);
+  return l;
+}
+
 static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
                                     hipStream_t s, PlanCtx *ctx, bool capturing) {
   int verbosity = gen->verbosity;
@@ -155,7 +176,8 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   }
   if (chunks >= (1ull << 32)) return mhcudaInvalidArguments;
   const size_t desc_bytes = sizeof(MatrixDesc) * count;
-  const size_t ctl_off = (desc_bytes + 255) & ~(size_t)255;
+  const PlanLayout layout = plan_layout(gen, mats);
+  const size_t ctl_off = layout.ctl_off;
   // While a graph is being captured nothing may allocate or wait here: the
   // graph's own staging is reserved and idle before capture starts.
   if (!capturing) {
@@ -163,7 +185,7 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
       HIPCH(hipEventCreateWithFlags(&ctx->copied, hipEventDisableTiming), mhcudaRuntimeError);
     HIPCH(hipEventSynchronize(ctx->copied), mhcudaRuntimeError);
   }
-  HIPCH(ctx->plan.reserve(ctl_off + sizeof(BatchCtl)), mhcudaMemoryAllocationFailure);
+  HIPCH(ctx->plan.reserve(layout.bytes), mhcudaMemoryAllocationFailure);
   if (ctx->cap < count) {
     if (ctx->desc_host) (void)hipHostFree(ctx->desc_host);
     ctx->desc_host = nullptr;
@@ -178,8 +200,8 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
         mhcudaMemoryCopyError);
   if (!capturing) HIPCH(hipEventRecord(ctx->copied, s), mhcudaRuntimeError);
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
-                          (uint32_t)chunks, (BatchCtl *)(dev + ctl_off), gen->tuning,
-                          gen->seg_auto,
+                          (uint32_t)chunks, (BatchCtl *)(dev + ctl_off),
+                          (StripedRow *)(dev + layout.list_off), gen->tuning, gen->seg_auto,
                           gen->num_cus, s, gen->timing && !capturing ? gen->ev_begin : nullptr,
                           gen->timing && !capturing ? gen->ev_end : nullptr),
         mhcudaRuntimeError);
@@ -235,6 +257,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
   const bool same = gc.exec && gc.kernel == kernel &&
                     gc.tuning.spl == gen->tuning.spl && gc.tuning.seg == gen->tuning.seg &&
                     gc.tuning.waves_per_cu == gen->tuning.waves_per_cu &&
+                    gc.tuning.stripe_mib == gen->tuning.stripe_mib &&
                     gc.key.size() == mats.size() &&
                     std::memcmp(gc.key.data(), mats.data(), sizeof(MatrixDesc) * mats.size()) == 0;
   if (!same) {
@@ -246,8 +269,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
     gc.release();
     // reserve the staging before capture: nothing allocates while capturing
     const size_t desc_bytes = sizeof(MatrixDesc) * mats.size();
-    HIPCH(gc.plan.plan.reserve(((desc_bytes + 255) & ~(size_t)255) + sizeof(BatchCtl)),
-          mhcudaMemoryAllocationFailure);
+    HIPCH(gc.plan.plan.reserve(plan_layout(gen, mats).bytes), mhcudaMemoryAllocationFailure);
     if (gc.plan.cap < mats.size()) {
       if (gc.plan.desc_host) (void)hipHostFree(gc.plan.desc_host);
       gc.plan.desc_host = nullptr;
@@ -720,6 +742,24 @@ MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen, int *samples_per
   if (samples_per_lane) *samples_per_lane = gen->tuning.spl;
   if (segment) *segment = gen->tuning.seg;
   if (waves_per_cu) *waves_per_cu = gen->tuning.waves_per_cu;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib) {
+  if (!gen || stripe_mib < 0) return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  gen->tuning.stripe_mib = stripe_mib;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib,
+                                uint32_t *stripes, uint32_t *min_nnz) {
+  if (!gen) return mhcudaInvalidArguments;
+  uint32_t k = 1, n = 0;
+  wmh::stripe_shape(gen->dim, gen->samples, gen->tuning.stripe_mib, &k, &n);
+  if (stripe_mib) *stripe_mib = gen->tuning.stripe_mib;
+  if (stripes) *stripes = k;
+  if (min_nnz) *min_nnz = k > 1 ? n : 0u;
   return mhcudaSuccess;
 }
 

@@ -31,13 +31,13 @@ struct MatrixDesc {
 
 // Control block of one persistent launch (device memory).
 struct BatchCtl {
-  uint32_t next_unit;     // work queue head
+  unsigned long long next_unit;  // work queue head (stripe pieces included)
   uint32_t total_units;   // items * sample tiles
   uint32_t total_items;
   uint32_t ntiles;
   uint32_t invalid;       // nonzero: some indptr decreases; nothing is hashed
   uint32_t seg;           // item size chosen by the plan kernel
-  uint32_t pad_[2];
+  uint32_t n_striped;     // striped rows listed by the init pass
 };
 
 // Kernel tuning knobs (chosen by the autotuner).
@@ -45,7 +45,44 @@ struct Tuning {
   int spl;          // samples per lane: 1, 2 or 4
   int seg;          // nonzeros per persistent work item (multiple of 64)
   int waves_per_cu; // persistent grid size = CUs * waves_per_cu
+  int stripe_mib;   // parameter bytes per column stripe in MiB (0: no stripes)
 };
+
+// Column stripes of the persistent launch.  When the parameter table is
+// larger than one stripe, every row with at least `min_nnz` nonzeros is cut
+// into `stripes` pieces of equal nonzero count; the work queue hands out all
+// first pieces, then all second pieces, and so on.  For rows whose columns
+// are sorted and spread over the table, piece k gathers from the k-th
+// `1 / stripes` of the table, so the waves in flight share a stripe of
+// parameters small enough to stay in the caches and address translation
+// caches.  Any row order is still hashed exactly: the pieces are merged like
+// rows split across work items.  Other rows are packed into items as before.
+struct StripedRow {
+  uint32_t matrix;
+  uint32_t row;
+};
+struct StripePlan {
+  uint32_t stripes;    // 1: no striping
+  uint32_t min_nnz;    // rows with at least this many nonzeros are striped
+  StripedRow *list;    // capacity: every row of the batch
+};
+
+// Stripe count and row threshold for a table of dim x samples parameters:
+// stripes of about `stripe_mib` MiB (at most kMaxStripes of them), pieces of
+// at least kStripePieceNnz nonzeros, rows of at least 1024.
+constexpr uint32_t kMaxStripes = 64;
+constexpr int kDefaultStripeMib = 32;
+constexpr uint32_t kStripePieceNnz = 128;
+inline void stripe_shape(uint32_t dim, uint32_t samples, int stripe_mib, uint32_t *stripes,
+                         uint32_t *min_nnz) {
+  const uint64_t table = (uint64_t)dim * 12u * padded_samples(samples);
+  const uint64_t stripe = (uint64_t)(stripe_mib > 0 ? stripe_mib : 0) << 20;
+  uint64_t k = stripe ? (table + stripe - 1) / stripe : 1;
+  if (k > kMaxStripes) k = kMaxStripes;
+  *stripes = k < 2 ? 1u : (uint32_t)k;
+  const uint32_t rows_min = *stripes * kStripePieceNnz;
+  *min_nnz = *stripes > 1 ? (rows_min > 1024u ? rows_min : 1024u) : 0xffffffffu;
+}
 
 hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
                              uint32_t seed, hipStream_t stream);
@@ -73,11 +110,14 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
 // kernel then sizes items for about four per resident wavefront).
 // `ev_begin` / `ev_end` (may be null) are recorded right around the
 // persistent hashing kernel.
+// `striped` (device memory, one entry per row of the batch) receives the
+// striped rows when tuning.stripe_mib asks for stripes; it may be null
+// otherwise.
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
-                             BatchCtl *ctl, const Tuning &tuning, bool auto_seg,
-                             int num_cus, hipStream_t stream, hipEvent_t ev_begin = nullptr,
-                             hipEvent_t ev_end = nullptr);
+                             BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
+                             bool auto_seg, int num_cus, hipStream_t stream,
+                             hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 // Weighted-Jaccard estimates of row pairs (pairs[2i], pairs[2i+1] index the
 // rows of hash tables a and b, each [rows][samples][2]).

@@ -404,13 +404,17 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
   }
 }
 
-// Hashes CSR positions [a, b) of matrix M (one item) for the lane's samples.
-// In a split row the argmin tracks the CSR position instead of the column.
+// Hashes CSR positions [a, b) of matrix M (one item or one stripe piece) for
+// the lane's samples.  In a split row the argmin tracks the CSR position
+// instead of the column.  `row` is the row containing a, or ~0u to search
+// for it.  Unless skip_nnz is 0, rows with at least `skip_nnz` nonzeros are
+// skipped (their stripe pieces hash them); skip_nnz >= 64.
 template <int V, int U>
 __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint32_t dim,
                                           uint32_t sp, uint32_t samples, uint32_t k0,
                                           LaneOffsets lo, const MatrixDesc &M,
-                                          uint32_t a, uint32_t b) {
+                                          uint32_t a, uint32_t b, uint32_t row,
+                                          uint32_t skip_nnz) {
   static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
   const uint32_t lane = lane_id();
   const char *pbase = reinterpret_cast<const char *>(params);
@@ -418,7 +422,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   const uint32_t *__restrict__ rows = M.rows;
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
-  uint32_t r = find_row(rows, M.length, a);
+  uint32_t r = row != ~0u ? row : find_row(rows, M.length, a);
   uint32_t row_start = uniform(rows[r]);
   // rows[rp_base + lane]: the next 64 row ends, read with one vector load
   uint32_t rp_base = r + 1;
@@ -426,6 +430,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   uint32_t rp_idx = 0;
   uint32_t row_end = readlane_u32(rp, 0);
   bool split = row_start < a || row_end > b;
+  bool skip = skip_nnz != 0u && row_end - row_start >= skip_nnz;
   Argmin<V> acc;
   acc.init();
 
@@ -443,10 +448,18 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
       row_end = readlane_u32(rp, rp_idx);
     } while (row_end == row_start);
     split = row_end > b;
+    skip = skip_nnz != 0u && row_end - row_start >= skip_nnz;
     acc.init();
   };
 
-  for (uint32_t cb = a; cb < b; cb += kWave) {
+  uint32_t cb = a;
+  while (cb < b) {
+    if (skip) {  // a striped row: jump past it
+      if (row_end >= b) break;
+      cb = row_end;
+      next_row();
+      continue;
+    }
     const uint32_t n = uniform(min((uint32_t)kWave, b - cb));
     uint32_t c = WMH_SENTINEL;
     float lw = 0.0f;
@@ -456,8 +469,9 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
     }
     // As in hash_range, the last group may run past n (slots >= n hold the
     // column WMH_SENTINEL and are never selected); only real positions can
-    // end a row.
-    for (uint32_t i = 0; i < n; i += U) {
+    // end a row.  A striped row starting inside the chunk runs past its end
+    // (it has at least 64 nonzeros): its slots are not folded.
+    for (uint32_t i = 0; i < n && !skip; i += U) {
       uint32_t j[U];
       float lwu[U];
       float rr[U][V], beta[U][V], lc[U][V];
@@ -470,18 +484,27 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
         load_v<V>(at(blk, lo.beta), beta[u]);
         load_v<V>(at(blk, lo.lc), lc[u]);
       }
+      const uint32_t p0 = cb + i;
+      if (row_end > p0 + min((uint32_t)U, n - i)) {
+        // the current row runs past the group: fold it without row checks
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const uint32_t p = cb + i + u;
-        fold<V>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u], j[u] < dim, acc);
-        if (i + u < n && p + 1 == row_end) {
-          flush_row<V>(M.output, r, samples, k0, split, acc);
-          if (p + 1 < b) next_row();
+        for (int u = 0; u < U; u++)
+          fold<V>(rr[u], beta[u], lc[u], split ? p0 + u : j[u], lwu[u], j[u] < dim, acc);
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const uint32_t p = p0 + u;
+          fold<V>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u], j[u] < dim && !skip, acc);
+          if (i + u < n && p + 1 == row_end) {
+            flush_row<V>(M.output, r, samples, k0, split, acc);
+            if (p + 1 < b) next_row();
+          }
         }
       }
     }
+    cb += n;
   }
-  if (row_end > b) flush_row<V>(M.output, r, samples, k0, true, acc);  // continues in a later item
+  if (row_end > b && !skip) flush_row<V>(M.output, r, samples, k0, true, acc);  // continues later
 }
 
 // Matrix whose range of `field` values contains x: the largest m with
@@ -501,29 +524,54 @@ __device__ __forceinline__ uint32_t find_matrix(const MatrixDesc *__restrict__ m
 template <int V>
 __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
-    const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl) {
+    const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
+    StripePlan stripes) {
   const uint32_t lane = lane_id();
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
-  const uint32_t total = uniform(ctl->total_units);
   const uint32_t ntiles = uniform(ctl->ntiles);
+  // stripe pieces first (all first pieces, then all second pieces, ...),
+  // then the items of the other rows
+  const uint64_t per_stripe = (uint64_t)uniform(ctl->n_striped) * ntiles;
+  const uint64_t piece_units = stripes.stripes > 1 ? per_stripe * stripes.stripes : 0ull;
+  const uint64_t total = piece_units + uniform(ctl->total_units);
+  const uint32_t skip_nnz = stripes.stripes > 1 ? stripes.min_nnz : 0u;
   for (;;) {
-    uint32_t u = 0;
-    if (lane == 0) u = atomicAdd(&ctl->next_unit, 1u);
-    u = readlane_u32(u, 0);
+    uint64_t u = 0;
+    if (lane == 0) u = atomicAdd(&ctl->next_unit, 1ull);
+    u = ((uint64_t)readlane_u32((uint32_t)(u >> 32), 0) << 32) | readlane_u32((uint32_t)u, 0);
     if (u >= total) break;  // every wave reaches this once the queue is drained
-    const uint32_t item = u / ntiles;
-    const uint32_t tile = u - item * ntiles;
-    const uint32_t m = find_matrix<&MatrixDesc::item_begin>(mats, count, item);
+    uint32_t tile, m, a, b, row;
+    if (u < piece_units) {
+      const uint32_t k = (uint32_t)(u / per_stripe);
+      const uint64_t rest = u - k * per_stripe;
+      const uint32_t i = (uint32_t)(rest / ntiles);
+      tile = (uint32_t)(rest - (uint64_t)i * ntiles);
+      const StripedRow sr = stripes.list[i];
+      m = uniform(sr.matrix);
+      row = uniform(sr.row);
+      const uint32_t s = uniform(mats[m].rows[row]);
+      const uint32_t n = uniform(mats[m].rows[row + 1]) - s;
+      a = s + (uint32_t)((uint64_t)n * k / stripes.stripes);
+      b = s + (uint32_t)((uint64_t)n * (k + 1) / stripes.stripes);
+      if (a == b) continue;
+    } else {
+      const uint64_t v = u - piece_units;
+      const uint32_t item = (uint32_t)(v / ntiles);
+      tile = (uint32_t)(v - (uint64_t)item * ntiles);
+      m = find_matrix<&MatrixDesc::item_begin>(mats, count, item);
+      const uint32_t rows0 = uniform(mats[m].rows[0]);
+      const uint32_t rend = uniform(mats[m].rows[mats[m].length]);
+      a = rows0 + (item - mats[m].item_begin) * seg;
+      b = a + min(seg, rend - a);
+      row = ~0u;
+    }
     const MatrixDesc M = mats[m];
-    const uint32_t rows0 = uniform(M.rows[0]);
-    const uint32_t rend = uniform(M.rows[M.length]);
-    const uint32_t a = rows0 + (item - M.item_begin) * seg;
-    const uint32_t b = a + min(seg, rend - a);
     const uint32_t k0 = tile * (kWave * V) + lane * V;
     const uint32_t koff = (k0 < samples) ? k0 : 0u;
     const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
-    hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b);
+    hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row,
+                                     row == ~0u ? skip_nnz : 0u);
   }
 }
 
@@ -591,6 +639,7 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
   }
   if (tid == 0) {
     ctl->next_unit = 0;
+    ctl->n_striped = 0;
     ctl->invalid = bad;
     ctl->seg = seg;
     ctl->total_items = carry;
@@ -600,19 +649,22 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
 }
 
 // Row scan over every matrix, 64 rows per wavefront.
-//   kInit:  empty rows and split rows get all-ones in every sample slot;
+//   kInit:  empty rows and split rows get all-ones in every sample slot, and
+//           rows to stripe are listed in stripes.list;
 //   kFixup: each split row's winning keys become (column, t).
+// A row is split when it crosses an item boundary or is striped.
 enum RowPass { kInit = 0, kFixup = 1 };
 
 template <int PASS>
 __global__ __launch_bounds__(kBlock) void batch_rows_kernel(
     const float *__restrict__ params, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, uint32_t total_chunks,
-    BatchCtl *__restrict__ ctl) {
+    BatchCtl *__restrict__ ctl, StripePlan stripes) {
   WMH_NO_CONTRACT
   const uint32_t lane = lane_id();
   if (PASS == kFixup && uniform(ctl->invalid)) return;
   const uint32_t seg = uniform(ctl->seg);
+  const uint32_t stripe_nnz = stripes.stripes > 1 ? stripes.min_nnz : 0xffffffffu;
   for (uint32_t w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); w < total_chunks;
        w += gridDim.x * kWavesPerBlock) {
     const uint32_t wv = uniform(w);
@@ -620,14 +672,27 @@ __global__ __launch_bounds__(kBlock) void batch_rows_kernel(
     const MatrixDesc M = mats[m];
     const uint32_t r0 = (wv - M.chunk_begin) * kWave;
     const uint32_t row = r0 + lane;
-    bool flag = false;
+    bool flag = false, striped = false;
     if (row < M.length) {
       const uint32_t rows0 = M.rows[0];
       const uint32_t s = M.rows[row], e = M.rows[row + 1];
       if (PASS == kInit && (e < s || s < rows0)) atomicOr(&ctl->invalid, 1u);
       const bool empty = s == e;
-      const bool split = !empty && (s - rows0) / seg != (e - 1u - rows0) / seg;
+      striped = e > s && e - s >= stripe_nnz;
+      const bool split = !empty && (striped || (s - rows0) / seg != (e - 1u - rows0) / seg);
       flag = PASS == kInit ? (empty || split) : split;
+    }
+    if (PASS == kInit) {
+      const unsigned long long smask = __ballot(striped);
+      if (smask) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->n_striped, (uint32_t)__builtin_popcountll(smask));
+        base = readlane_u32(base, 0);
+        if (striped) {
+          const uint32_t rank = (uint32_t)__builtin_popcountll(smask & ((1ull << lane) - 1ull));
+          stripes.list[base + rank] = StripedRow{m, row};
+        }
+      }
     }
     unsigned long long mask = __ballot(flag);
     while (mask) {
@@ -841,13 +906,16 @@ hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t tot
 
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
-                             BatchCtl *ctl, const Tuning &tuning, bool auto_seg,
-                             int num_cus, hipStream_t stream, hipEvent_t ev_begin,
-                             hipEvent_t ev_end) {
+                             BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
+                             bool auto_seg, int num_cus, hipStream_t stream,
+                             hipEvent_t ev_begin, hipEvent_t ev_end) {
   const uint32_t sp = padded_samples(samples);
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
+  StripePlan stripes{1u, 0xffffffffu, striped};
+  stripe_shape(dim, samples, tuning.stripe_mib, &stripes.stripes, &stripes.min_nnz);
+  if (stripes.stripes > 1 && striped == nullptr) return hipErrorInvalidValue;
   const uint32_t seg = (uint32_t)tuning.seg;
   const uint32_t target = auto_seg ? (uint32_t)std::max(1, num_cus * tuning.waves_per_cu * 4) : 0u;
   const uint32_t per_wave = (uint32_t)(kWave * spl);
@@ -856,26 +924,26 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
                      target, ntiles, ctl);
   const unsigned row_grid = grid_for(total_chunks);
   hipLaunchKernelGGL(batch_rows_kernel<kInit>, dim3(row_grid), dim3(kBlock), 0, stream,
-                     params, samples, sp, mats, count, total_chunks, ctl);
+                     params, samples, sp, mats, count, total_chunks, ctl, stripes);
   const unsigned grid = (unsigned)std::max(1, num_cus * tuning.waves_per_cu / kWavesPerBlock);
   if (ev_begin) (void)hipEventRecord(ev_begin, stream);
   switch (spl) {
     case 1:
       hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, ctl);
+                         params, dim, samples, sp, mats, count, ctl, stripes);
       break;
     case 2:
       hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, ctl);
+                         params, dim, samples, sp, mats, count, ctl, stripes);
       break;
     default:
       hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, ctl);
+                         params, dim, samples, sp, mats, count, ctl, stripes);
       break;
   }
   if (ev_end) (void)hipEventRecord(ev_end, stream);
   hipLaunchKernelGGL(batch_rows_kernel<kFixup>, dim3(row_grid), dim3(kBlock), 0, stream,
-                     params, samples, sp, mats, count, total_chunks, ctl);
+                     params, samples, sp, mats, count, total_chunks, ctl, stripes);
   return hipGetLastError();
 }
 

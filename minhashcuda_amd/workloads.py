@@ -8,23 +8,30 @@
   lognormal weights (64 matrices, dim 2^20, rows in [1k, 200k], Zipf(1.2)
   clipped to [1, 20000], lognormal(0, 1.5), 128 samples).
 
-Columns of a row with n nonzeros are drawn stratified -- the i-th one
-uniformly in [i*dim/n, (i+1)*dim/n) -- which makes them distinct and sorted
-in one vectorised pass; each column is still uniform over its stratum.
+Columns of a row with n nonzeros are n distinct sorted values, close to a
+uniform n-subset of [0, dim): n draws x uniform in [0, dim - n] are sorted
+within the row and the i-th gets i added (which makes them distinct).  Rows of
+equal length therefore share no column pattern: a generator that placed the
+i-th column of every row in the same stratum [i*dim/n, (i+1)*dim/n) would make
+equal-length rows gather the same parameter blocks at the same time, a cache
+reuse that real inputs do not have.
 """
 import numpy as np
 import scipy.sparse as sp
 
 
-def _stratified_columns(indptr, dim, u):
-    """Columns for every position given row offsets and uniforms u in [0,1)."""
+def _uniform_columns(indptr, dim, u):
+    """Columns for every position given row offsets and uniforms u in [0,1)
+    (a row with more nonzeros than dim gets sorted columns with repeats)."""
     nnz = np.diff(indptr)
     total = int(indptr[-1])
-    pos = np.arange(total, dtype=np.int64) - np.repeat(indptr[:-1], nnz)
+    row = np.repeat(np.arange(len(nnz), dtype=np.int64), nnz)
     n = np.repeat(nnz, nnz).astype(np.int64)
-    lo = pos * dim // np.maximum(n, 1)
-    hi = (pos + 1) * dim // np.maximum(n, 1)
-    return (lo + (u * (hi - lo)).astype(np.int64)).astype(np.uint32)
+    distinct = n <= dim
+    x = (u * np.where(distinct, dim - n + 1, dim)).astype(np.int64)
+    key = np.sort(row * (dim + 1) + x)  # sorted within each row, rows stay in place
+    pos = np.arange(total, dtype=np.int64) - np.repeat(indptr[:-1], nnz)
+    return (key - row * (dim + 1) + np.where(distinct, pos, 0)).astype(np.uint32)
 
 
 def plain_csr(rows=100000, dim=1 << 20, mean=100.0, sigma=1.0, seed=1):
@@ -32,7 +39,7 @@ def plain_csr(rows=100000, dim=1 << 20, mean=100.0, sigma=1.0, seed=1):
     nnz = np.minimum(rng.poisson(mean, rows), dim)
     indptr = np.zeros(rows + 1, np.int64)
     np.cumsum(nnz, out=indptr[1:])
-    cols = _stratified_columns(indptr, dim, rng.random(int(indptr[-1])))
+    cols = _uniform_columns(indptr, dim, rng.random(int(indptr[-1])))
     w = rng.lognormal(0.0, sigma, int(indptr[-1])).astype(np.float32)
     return sp.csr_matrix((w, cols, indptr.astype(np.uint32)), shape=(rows, dim))
 
@@ -51,7 +58,7 @@ def batched_csr(nnz_per_row, dim=1 << 20, sigma=1.5, seed=1):
     indptr = np.zeros(len(nnz_per_row) + 1, np.int64)
     np.cumsum(nnz_per_row, out=indptr[1:])
     assert indptr[-1] < 2 ** 32
-    cols = _stratified_columns(indptr, dim, rng.random(int(indptr[-1])))
+    cols = _uniform_columns(indptr, dim, rng.random(int(indptr[-1])))
     w = rng.lognormal(0.0, sigma, int(indptr[-1])).astype(np.float32)
     return sp.csr_matrix((w, cols, indptr.astype(np.uint32)), shape=(len(nnz_per_row), dim))
 
@@ -66,31 +73,41 @@ def batched_csr_torch(nnz_per_row, dim=1 << 20, sigma=1.5, seed=1, device="cuda"
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(int(seed))
-    nnz = torch.as_tensor(nnz_per_row, dtype=torch.int64, device=device)
-    indptr = torch.zeros(len(nnz_per_row) + 1, dtype=torch.int64, device=device)
+    nnz_host = np.asarray(nnz_per_row, dtype=np.int64)
+    nnz = torch.as_tensor(nnz_host, device=device)
+    indptr = torch.zeros(len(nnz_host) + 1, dtype=torch.int64, device=device)
     torch.cumsum(nnz, 0, out=indptr[1:])
-    total = int(indptr[-1].item())
+    ip_host = np.zeros(len(nnz_host) + 1, np.int64)
+    np.cumsum(nnz_host, out=ip_host[1:])
+    total = int(ip_host[-1])
     assert total < 2 ** 32
     cols = torch.empty(total, dtype=torch.int32, device=device)
     vals = torch.empty(total, dtype=torch.float32, device=device)
     step = 1 << 27
-    row_of = torch.repeat_interleave(torch.arange(len(nnz_per_row), device=device), nnz)
-    for s in range(0, total, step):
-        e = min(total, s + step)
-        r = row_of[s:e]
-        pos = torch.arange(s, e, dtype=torch.int64, device=device) - indptr[r]
-        n = nnz[r]
-        lo = pos * dim // n
-        hi = (pos + 1) * dim // n
+    r0 = 0
+    while r0 < len(nnz_host):  # whole rows, about `step` nonzeros at a time
+        r1 = int(np.searchsorted(ip_host, ip_host[r0] + step, side="right")) - 1
+        r1 = min(max(r1, r0 + 1), len(nnz_host))
+        s, e = int(ip_host[r0]), int(ip_host[r1])
+        r0_, r1_ = r0, r1
+        r0 = r1
+        if e == s:
+            continue
+        row = torch.repeat_interleave(torch.arange(r1_ - r0_, device=device), nnz[r0_:r1_])
+        n = nnz[r0_:r1_][row]
+        distinct = n <= dim  # longer rows get sorted columns with repeats
         u = torch.rand(e - s, generator=g, device=device, dtype=torch.float64)
-        cols[s:e] = (lo + (u * (hi - lo).to(torch.float64)).to(torch.int64)).to(torch.int32)
+        x = (u * torch.where(distinct, dim - n + 1, dim).to(torch.float64)).to(torch.int64)
+        key = torch.sort(row * (dim + 1) + x).values
+        pos = torch.arange(s, e, dtype=torch.int64, device=device) - indptr[r0_:r1_][row]
+        cols[s:e] = (key - row * (dim + 1) + torch.where(distinct, pos, 0)).to(torch.int32)
+        del row, n, u, x, key, pos, distinct
         if weights == "uniform100":
             u2 = torch.rand(e - s, generator=g, device=device, dtype=torch.float32)
             vals[s:e] = (1.0 - u2) * 100.0
         else:
             z = torch.randn(e - s, generator=g, device=device, dtype=torch.float32)
             vals[s:e] = torch.exp(z * sigma)
-    del row_of
     return vals, cols, indptr.to(torch.int64).to(torch.int32)
 
 

@@ -71,6 +71,46 @@ def test_persistent_matches_oracle(libmhcuda, capi, batch, expected, seg, spl, w
         assert np.array_equal(a, r), "matrix %d (second call)" % i
 
 
+def shuffled_rows(m, seed):
+    """The same rows with their nonzeros in random order."""
+    rng = np.random.default_rng(seed)
+    m = m.copy()
+    for r in range(m.shape[0]):
+        s, e = m.indptr[r], m.indptr[r + 1]
+        perm = rng.permutation(e - s)
+        m.indices[s:e] = m.indices[s:e][perm]
+        m.data[s:e] = m.data[s:e][perm]
+    return m
+
+
+@pytest.mark.parametrize("stripe_mib,seg,spl", [(8, 512, 2), (2, 256, 1), (3, 2048, 4)])
+@pytest.mark.parametrize("shuffled", [False, True])
+def test_column_stripes(libmhcuda, capi, batch, expected, stripe_mib, seg, spl, shuffled):
+    """Rows of many nonzeros cut into column-stripe pieces (sorted columns,
+    and the same rows in random order): every hash equals the oracle's."""
+    say("stripes", stripe_mib, "MiB seg", seg, "spl", spl, "shuffled", shuffled)
+    params, ref = expected
+    mats = [shuffled_rows(m, i) for i, m in enumerate(batch)] if shuffled else batch
+    if shuffled:
+        ref = [ref_hashes(params, m) for m in mats]
+    gen = libmhcuda.minhash_cuda_init(1 << 14, 128, deferred=True)
+    libmhcuda.minhash_cuda_assign_vars(gen, *params)
+    g = ctypes.c_void_p(gen)
+    assert capi.mhcuda_set_tuning(g, spl, seg, 0) == 0
+    assert capi.mhcuda_set_stripes(g, stripe_mib) == 0
+    mib, k, min_nnz = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_uint32()
+    assert capi.mhcuda_get_stripes(g, ctypes.byref(mib), ctypes.byref(k), ctypes.byref(min_nnz)) == 0
+    assert mib.value == stripe_mib and k.value > 1
+    striped = sum(int((np.diff(m.indptr) >= min_nnz.value).sum()) for m in mats)
+    assert striped > 0
+    got = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
+    again = libmhcuda.minhash_cuda_calc_batch(gen, mats[::-1], kernel=PERSISTENT)[::-1]
+    libmhcuda.minhash_cuda_fini(gen)
+    for i, (a, b, r) in enumerate(zip(got, again, ref)):
+        assert np.array_equal(a, r), "matrix %d" % i
+        assert np.array_equal(b, r), "matrix %d (second call)" % i
+
+
 def test_plain_batch_matches_oracle(libmhcuda, batch, expected):
     params, ref = expected
     gen = libmhcuda.minhash_cuda_init(1 << 14, 128, deferred=True)

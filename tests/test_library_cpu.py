@@ -77,6 +77,8 @@ def test_null_generator_paths(capi):
     assert capi.mhcuda_calc_batch(None, 1, None, 0) == 1
     assert capi.mhcuda_fini(None) == 0
     assert capi.mhcuda_set_tuning(None, 2, 0, 0) == 1
+    assert capi.mhcuda_set_stripes(None, 512) == 1
+    assert capi.mhcuda_get_stripes(None, None, None, None) == 1
 
 
 def test_python_module_surface(libmhcuda):
