@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--stripe-mib", type=int, default=-1,
                    help="persistent column-stripe size in MiB (0: off, -1: library default)")
     p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
+    p.add_argument("--kernel", default="plain", choices=["plain", "persistent"],
+                   help="plain workload: the kernel to time (default: the plain kernel)")
     p.add_argument("--no-graph", action="store_true",
                    help="launch eagerly instead of replaying a captured hipGraph")
     p.add_argument("--no-autotune", action="store_true",
@@ -160,14 +162,16 @@ def measured_traffic(workload, kernel, alg_bytes, stripe_mib=None):
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f)["workloads"][workload]
+            recs = json.load(f)["workloads"][workload]
     except (OSError, KeyError, ValueError):
         return None
-    if rec.get("kernel") != kernel or rec.get("algorithmic_bytes") != alg_bytes:
-        return None
-    if stripe_mib is not None and rec.get("stripe_mib") != stripe_mib:
-        return None
-    return rec["traffic_bytes"]
+    for rec in recs:
+        if rec.get("kernel") != kernel or rec.get("algorithmic_bytes") != alg_bytes:
+            continue
+        if stripe_mib is not None and rec.get("stripe_mib") != stripe_mib:
+            continue
+        return rec["traffic_bytes"]
+    return None
 
 
 def timed_steps(torch, run, engine, steps, warmup):
@@ -498,10 +502,14 @@ def main():
         tensors = [(torch.from_numpy(m.data.astype(np.float32)).cuda(),
                     torch.from_numpy(m.indices.astype(np.int32)).cuda(),
                     torch.from_numpy(m.indptr.astype(np.int64).astype(np.int32)).cuda())]
-        kernel = cm.KERNEL_PLAIN
-        kname = "wmh_plain_kernel"
+        if args.kernel == "persistent":
+            kernel = cm.KERNEL_PERSISTENT
+            kname = "wmh_persistent_kernel"
+        else:
+            kernel = cm.KERNEL_PLAIN
+            kname = "wmh_plain_kernel"
         desc = ("plain: 100000 x 2^20 CSR, nnz/row~Poisson(100), lognormal(0,1) weights, "
-                "128 samples, one launch (plain kernel)")
+                "128 samples, one launch (%s kernel)" % args.kernel)
     else:
         steps = args.steps or 3
         warmup = args.warmup if args.warmup is not None else 1

@@ -71,7 +71,7 @@ struct StripePlan {
 // stripes of about `stripe_mib` MiB (at most kMaxStripes of them), pieces of
 // at least kStripePieceNnz nonzeros, rows of at least 1024.
 constexpr uint32_t kMaxStripes = 64;
-constexpr int kDefaultStripeMib = 32;
+constexpr int kDefaultStripeMib = 16;
 constexpr uint32_t kStripePieceNnz = 128;
 inline void stripe_shape(uint32_t dim, uint32_t samples, int stripe_mib, uint32_t *stripes,
                          uint32_t *min_nnz) {
