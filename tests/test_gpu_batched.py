@@ -177,3 +177,70 @@ def test_pipelined_host_path_many_pieces(libmhcuda, batch, expected, monkeypatch
     for g, r in zip(got, ref):
         assert np.array_equal(g, r)
     assert np.array_equal(h, ref[2][11:])
+
+
+def test_stripes_edge_values(libmhcuda, capi):
+    """Striped rows holding the edge cases: zero, negative, NaN, infinite and
+    subnormal weights, columns >= dim, a row of exactly the striping threshold,
+    an all-zero striped row, and a tie between two columns with identical
+    parameters whose positions fall in different stripe pieces (the earlier
+    position must win, as in the sequential walk)."""
+    dim, samples = 4000, 64
+    rng = np.random.default_rng(17)
+    rs = rng.gamma(2.0, 1.0, (samples, dim)).astype(np.float32)
+    ln_cs = np.log(rng.gamma(2.0, 1.0, (samples, dim))).astype(np.float32)
+    betas = rng.uniform(0, 1, (samples, dim)).astype(np.float32)
+    j1, j2 = 1500, 1501
+    for t in (rs, ln_cs, betas):
+        t[:, j2] = t[:, j1]
+
+    def row_sorted(n, hi=dim):
+        return np.sort(rng.choice(hi, n, replace=False)).astype(np.uint32)
+
+    rows_cols, rows_w = [], []
+    # A: 3000 nonzeros, stripe pieces of 1000; the tied pair sits at positions 999 | 1000,
+    # and the last columns are >= dim
+    below = np.sort(rng.choice(j1, 999, replace=False))
+    above = np.sort(rng.choice(np.arange(j2 + 1, dim + 10), 1999, replace=False))
+    c = np.concatenate([below, [j1, j2], above]).astype(np.uint32)
+    assert c[999] == j1 and c[1000] == j2 and c[-1] >= dim
+    w = rng.lognormal(0, 1, 3000).astype(np.float32)
+    w[999] = w[1000] = np.float32(1e30)
+    w[10], w[11], w[12], w[13] = 0.0, -1.0, np.nan, np.float32(1e-42)
+    rows_cols.append(c)
+    rows_w.append(w)
+    # E: striped, an infinite weight in its last piece and a subnormal one
+    c = row_sorted(1500)
+    w = rng.lognormal(0, 1, 1500).astype(np.float32)
+    w[1400], w[3] = np.inf, np.float32(1e-41)
+    rows_cols.append(c)
+    rows_w.append(w)
+    rows_cols.append(np.array([], np.uint32)); rows_w.append(np.array([], np.float32))
+    # B: exactly the threshold, unsorted columns
+    c = rng.permutation(row_sorted(1024)).astype(np.uint32)
+    rows_cols.append(c); rows_w.append(rng.lognormal(0, 1, 1024).astype(np.float32))
+    # C: one below the threshold
+    rows_cols.append(row_sorted(1023)); rows_w.append(rng.lognormal(0, 1, 1023).astype(np.float32))
+    # D: striped, every weight zero
+    rows_cols.append(row_sorted(2000)); rows_w.append(np.zeros(2000, np.float32))
+    for n in (1, 5, 60):
+        rows_cols.append(row_sorted(n)); rows_w.append(rng.lognormal(0, 1, n).astype(np.float32))
+    indptr = np.concatenate([[0], np.cumsum([len(c) for c in rows_cols])]).astype(np.uint32)
+    m = sp.csr_matrix((np.concatenate(rows_w), np.concatenate(rows_cols), indptr),
+                      shape=(len(rows_cols), dim + 10))
+    m.indices = np.concatenate(rows_cols).astype(np.uint32)  # keep out-of-range columns
+    ref = oracle.calc(rs, ln_cs, betas, m.data, m.indices, m.indptr)
+    # the tie resolves to the earlier column wherever the pair wins
+    assert (ref[0, :, 0] == j1).any() and not (ref[0, :, 0] == j2).any()
+    gen = libmhcuda.minhash_cuda_init(dim, samples, deferred=True)
+    libmhcuda.minhash_cuda_assign_vars(gen, rs, ln_cs, betas)
+    g = ctypes.c_void_p(gen)
+    assert capi.mhcuda_set_stripes(g, 1) == 0
+    k, min_nnz = ctypes.c_uint32(), ctypes.c_uint32()
+    assert capi.mhcuda_get_stripes(g, None, ctypes.byref(k), ctypes.byref(min_nnz)) == 0
+    assert k.value == 3 and min_nnz.value == 1024
+    for seg in (64, 1024):
+        assert capi.mhcuda_set_tuning(g, 0, seg, 0) == 0
+        got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+        assert np.array_equal(got, ref), seg
+    libmhcuda.minhash_cuda_fini(gen)
