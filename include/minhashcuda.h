@@ -170,10 +170,10 @@ typedef struct {
  * matrices: measures the batch's nonzero-weighted row-length histogram on the
  * device, and unless a batch of the same shape (histogram bins of the median
  * and 90th-percentile nonzero, size class, samples) was tuned before, times
- * candidates on a prefix of up to 2^28 nonzeros (kernel, samples per lane,
- * item size, waves per CU, column-stripe size).  The choice becomes the
- * generator's tuning (kernel mhcudaKernelAuto then uses it).  The outputs of
- * the timed prefix are overwritten. */
+ * candidates (kernel, samples per lane, item size, waves per CU, column-stripe
+ * size) on a prefix of the batch holding up to 2^32 nonzeros.  The choice
+ * becomes the generator's tuning (kernel mhcudaKernelAuto then uses it).  The
+ * timed runs write the prefix's hashes into its outputs. */
 MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t count,
                              const MHCUDAMatrix *mats, void *stream,
                              MHCUDATuneResult *result);

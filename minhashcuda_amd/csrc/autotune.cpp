@@ -8,9 +8,12 @@
 // device.  On a cache miss the tuner times candidates on a bounded prefix of
 // the batch with HIP events, by coordinate descent: kernel (plain or
 // persistent), samples per lane, then -- for the persistent kernel -- the
-// work-item size and the waves per CU.  The winner becomes the generator's
-// tuning and is cached under the key, so a later batch of the same shape is
-// tuned without timing anything.
+// work-item size, the waves per CU and the column-stripe size.  The prefix
+// holds up to 2^32 nonzeros (the whole batch when smaller): the gain of column
+// stripes depends on how often the batch reuses each stripe of the table,
+// which a small prefix understates.  The winner becomes the generator's tuning
+// and is cached under the key, so a later batch of the same shape is tuned
+// without timing anything.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,7 +26,11 @@
 
 namespace {
 
-constexpr uint64_t kSampleNnz = 1ull << 28;  // nonzeros timed per candidate
+// Nonzeros timed per candidate: a prefix of the batch of at most this size.
+// It is large on purpose: column stripes and the lane width interact with how
+// often the batch reuses each part of the parameter table, which a small
+// prefix understates.
+constexpr uint64_t kSampleNnz = 1ull << 32;
 
 struct Candidate {
   int kernel;
@@ -106,16 +113,22 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
 
   auto it = gen->tuned.find(key);
   if (it == gen->tuned.end()) {
-    // the timed prefix: whole matrices, the last one cut to a row prefix
-    std::vector<wmh::MatrixDesc> sample;
-    uint64_t budget = kSampleNnz, have = 0;
-    for (uint32_t m = 0; m < count && have < budget; m++) {
+    // nonzeros of every matrix
+    std::vector<uint64_t> nnz_of(count, 0);
+    for (uint32_t m = 0; m < count; m++) {
       if (descs[m].length == 0) continue;
       uint32_t r0 = 0, r1 = 0;
       if (hipMemcpy(&r0, descs[m].rows, 4, hipMemcpyDeviceToHost) != hipSuccess ||
           hipMemcpy(&r1, descs[m].rows + descs[m].length, 4, hipMemcpyDeviceToHost) != hipSuccess)
         return mhcudaMemoryCopyError;
-      const uint64_t n = r1 >= r0 ? r1 - r0 : 0;
+      nnz_of[m] = r1 >= r0 ? r1 - r0 : 0;
+    }
+    // the timed prefix: whole matrices, the last one cut to a row prefix
+    std::vector<wmh::MatrixDesc> sample;
+    uint64_t budget = kSampleNnz, have = 0;
+    for (uint32_t m = 0; m < count && have < budget; m++) {
+      if (descs[m].length == 0) continue;
+      const uint64_t n = nnz_of[m];
       wmh::MatrixDesc d = descs[m];
       d.chunk_begin = 0;
       if (have + n > budget && n > 0) {
@@ -132,9 +145,10 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
     const wmh::Tuning saved = gen->tuning;
     const bool saved_timing = gen->timing;
     gen->timing = false;
-    auto time_one = [&](const Candidate &c, float *ms) -> MHCUDAResult {
+    auto time_on = [&](const std::vector<wmh::MatrixDesc> &batch, const Candidate &c,
+                       float *ms) -> MHCUDAResult {
       gen->tuning = c.tuning;
-      std::vector<wmh::MatrixDesc> d = sample;
+      std::vector<wmh::MatrixDesc> d = batch;
       if (hipEventRecord(e0, s) != hipSuccess) return mhcudaRuntimeError;
       MHCUDAResult r = wmh::hash_device_batch(gen, d, c.kernel, s, &gen->plan, false);
       if (r != mhcudaSuccess) return r;
@@ -143,6 +157,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
         return mhcudaRuntimeError;
       return mhcudaSuccess;
     };
+    auto time_one = [&](const Candidate &c, float *ms) { return time_on(sample, c, ms); };
     Candidate best{mhcudaKernelPersistent, saved};
     float best_ms = 0;
     MHCUDAResult r = time_one(best, &best_ms);  // warm-up
@@ -182,7 +197,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
       // column stripes (only where the table spans several stripes)
       const Candidate after_wpc = best;
       uint32_t k = 1, min_nnz = 0;
-      for (int mib : {0, 16, 32, 64, 256}) {
+      for (int mib : {0, 4, 16, 64, 256}) {
         if (mib == after_wpc.tuning.stripe_mib) continue;
         wmh::stripe_shape(gen->dim, gen->samples, mib, &k, &min_nnz);
         if (mib != 0 && k < 2) continue;

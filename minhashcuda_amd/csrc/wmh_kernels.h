@@ -64,15 +64,16 @@ struct StripedRow {
 struct StripePlan {
   uint32_t stripes;    // 1: no striping
   uint32_t min_nnz;    // rows with at least this many nonzeros are striped
+  uint32_t per_unit;   // pieces handed out per queue unit
   StripedRow *list;    // capacity: every row of the batch
 };
 
 // Stripe count and row threshold for a table of dim x samples parameters:
 // stripes of about `stripe_mib` MiB (at most kMaxStripes of them), pieces of
 // at least kStripePieceNnz nonzeros, rows of at least 1024.
-constexpr uint32_t kMaxStripes = 64;
-constexpr int kDefaultStripeMib = 16;
-constexpr uint32_t kStripePieceNnz = 128;
+constexpr uint32_t kMaxStripes = 256;
+constexpr int kDefaultStripeMib = 4;
+constexpr uint32_t kStripePieceNnz = 32;
 inline void stripe_shape(uint32_t dim, uint32_t samples, int stripe_mib, uint32_t *stripes,
                          uint32_t *min_nnz) {
   const uint64_t table = (uint64_t)dim * 12u * padded_samples(samples);

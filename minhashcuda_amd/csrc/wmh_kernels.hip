@@ -530,9 +530,12 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
   const uint32_t ntiles = uniform(ctl->ntiles);
-  // stripe pieces first (all first pieces, then all second pieces, ...),
-  // then the items of the other rows
-  const uint64_t per_stripe = (uint64_t)uniform(ctl->n_striped) * ntiles;
+  // stripe pieces first (all first pieces, then all second pieces, ...; a
+  // unit is `per_unit` consecutive pieces of one stripe), then the items of
+  // the other rows
+  const uint64_t pieces = (uint64_t)uniform(ctl->n_striped) * ntiles;  // per stripe
+  const uint32_t per_unit = stripes.per_unit;
+  const uint64_t per_stripe = (pieces + per_unit - 1) / per_unit;
   const uint64_t piece_units = stripes.stripes > 1 ? per_stripe * stripes.stripes : 0ull;
   const uint64_t total = piece_units + uniform(ctl->total_units);
   const uint32_t skip_nnz = stripes.stripes > 1 ? stripes.min_nnz : 0u;
@@ -541,37 +544,45 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     if (lane == 0) u = atomicAdd(&ctl->next_unit, 1ull);
     u = ((uint64_t)readlane_u32((uint32_t)(u >> 32), 0) << 32) | readlane_u32((uint32_t)u, 0);
     if (u >= total) break;  // every wave reaches this once the queue is drained
-    uint32_t tile, m, a, b, row;
-    if (u < piece_units) {
-      const uint32_t k = (uint32_t)(u / per_stripe);
-      const uint64_t rest = u - k * per_stripe;
-      const uint32_t i = (uint32_t)(rest / ntiles);
-      tile = (uint32_t)(rest - (uint64_t)i * ntiles);
-      const StripedRow sr = stripes.list[i];
-      m = uniform(sr.matrix);
-      row = uniform(sr.row);
-      const uint32_t s = uniform(mats[m].rows[row]);
-      const uint32_t n = uniform(mats[m].rows[row + 1]) - s;
-      a = s + (uint32_t)((uint64_t)n * k / stripes.stripes);
-      b = s + (uint32_t)((uint64_t)n * (k + 1) / stripes.stripes);
-      if (a == b) continue;
-    } else {
-      const uint64_t v = u - piece_units;
-      const uint32_t item = (uint32_t)(v / ntiles);
-      tile = (uint32_t)(v - (uint64_t)item * ntiles);
-      m = find_matrix<&MatrixDesc::item_begin>(mats, count, item);
-      const uint32_t rows0 = uniform(mats[m].rows[0]);
-      const uint32_t rend = uniform(mats[m].rows[mats[m].length]);
-      a = rows0 + (item - mats[m].item_begin) * seg;
-      b = a + min(seg, rend - a);
-      row = ~0u;
+    const bool piece = u < piece_units;
+    uint32_t k = 0;
+    uint64_t q = 0, q_end = 1;
+    if (piece) {
+      k = (uint32_t)(u / per_stripe);
+      q = (u - k * per_stripe) * per_unit;
+      q_end = min(q + per_unit, pieces);
     }
-    const MatrixDesc M = mats[m];
-    const uint32_t k0 = tile * (kWave * V) + lane * V;
-    const uint32_t koff = (k0 < samples) ? k0 : 0u;
-    const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
-    hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row,
-                                     row == ~0u ? skip_nnz : 0u);
+    for (; q < q_end; q++) {
+      uint32_t tile, m, a, b, row;
+      if (piece) {
+        const uint32_t i = (uint32_t)(q / ntiles);
+        tile = (uint32_t)(q - (uint64_t)i * ntiles);
+        const StripedRow sr = stripes.list[i];
+        m = uniform(sr.matrix);
+        row = uniform(sr.row);
+        const uint32_t s = uniform(mats[m].rows[row]);
+        const uint32_t n = uniform(mats[m].rows[row + 1]) - s;
+        a = s + (uint32_t)((uint64_t)n * k / stripes.stripes);
+        b = s + (uint32_t)((uint64_t)n * (k + 1) / stripes.stripes);
+        if (a == b) continue;
+      } else {
+        const uint64_t v = u - piece_units;
+        const uint32_t item = (uint32_t)(v / ntiles);
+        tile = (uint32_t)(v - (uint64_t)item * ntiles);
+        m = find_matrix<&MatrixDesc::item_begin>(mats, count, item);
+        const uint32_t rows0 = uniform(mats[m].rows[0]);
+        const uint32_t rend = uniform(mats[m].rows[mats[m].length]);
+        a = rows0 + (item - mats[m].item_begin) * seg;
+        b = a + min(seg, rend - a);
+        row = ~0u;
+      }
+      const MatrixDesc M = mats[m];
+      const uint32_t k0 = tile * (kWave * V) + lane * V;
+      const uint32_t koff = (k0 < samples) ? k0 : 0u;
+      const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
+      hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row,
+                                       row == ~0u ? skip_nnz : 0u);
+    }
   }
 }
 
@@ -913,8 +924,10 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
-  StripePlan stripes{1u, 0xffffffffu, striped};
+  StripePlan stripes{1u, 0xffffffffu, 1u, striped};
   stripe_shape(dim, samples, tuning.stripe_mib, &stripes.stripes, &stripes.min_nnz);
+  // about half an item's nonzeros of the shortest pieces per queue unit
+  stripes.per_unit = std::max(1u, (uint32_t)tuning.seg / (2u * kStripePieceNnz));
   if (stripes.stripes > 1 && striped == nullptr) return hipErrorInvalidValue;
   const uint32_t seg = (uint32_t)tuning.seg;
   const uint32_t target = auto_seg ? (uint32_t)std::max(1, num_cus * tuning.waves_per_cu * 4) : 0u;
