@@ -391,12 +391,18 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
   if (split) {
     unsigned long long *slot =
         reinterpret_cast<unsigned long long *>(output) + ((uint64_t)row * samples + k0);
+    // A slot only decreases, so a value read here is >= its current one: when
+    // it is already <= key the atomic would change nothing and is skipped
+    // (most pieces of a striped row do not improve on the earlier ones).
+    unsigned long long seen[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) seen[v] = k0 + v < samples ? slot[v] : 0ull;
 #pragma unroll
     for (int v = 0; v < V; v++) {
       if (k0 + v < samples && acc.j[v] != WMH_SENTINEL) {
         const unsigned long long key =
             ((unsigned long long)order_key(acc.best[v]) << 32) | acc.j[v];
-        atomicMin(slot + v, key);
+        if (key < seen[v]) atomicMin(slot + v, key);
       }
     }
   } else {
@@ -534,7 +540,11 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   // unit is `per_unit` consecutive pieces of one stripe), then the items of
   // the other rows
   const uint64_t pieces = (uint64_t)uniform(ctl->n_striped) * ntiles;  // per stripe
-  const uint32_t per_unit = stripes.per_unit;
+  // at most stripes.per_unit, and few enough that every stripe has several
+  // units per wavefront of the grid (so the waves in flight share a stripe)
+  const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
+  const uint32_t per_unit =
+      (uint32_t)max(1ull, min((unsigned long long)stripes.per_unit, pieces / (4ull * waves)));
   const uint64_t per_stripe = (pieces + per_unit - 1) / per_unit;
   const uint64_t piece_units = stripes.stripes > 1 ? per_stripe * stripes.stripes : 0ull;
   const uint64_t total = piece_units + uniform(ctl->total_units);
