@@ -30,8 +30,11 @@ struct MatrixDesc {
 };
 
 // Control block of one persistent launch (device memory).
+// Work queues of the persistent kernel: one per XCD for stripe pieces (the
+// stripes k with k % kXcds == q go to queue q), then one for the items.
+constexpr uint32_t kXcds = 8;
 struct BatchCtl {
-  unsigned long long next_unit;  // work queue head (stripe pieces included)
+  unsigned long long next[kXcds + 1];  // queue heads
   uint32_t total_units;   // items * sample tiles
   uint32_t total_items;
   uint32_t ntiles;

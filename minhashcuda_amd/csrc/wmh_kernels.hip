@@ -536,62 +536,74 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
   const uint32_t ntiles = uniform(ctl->ntiles);
-  // stripe pieces first (all first pieces, then all second pieces, ...; a
-  // unit is `per_unit` consecutive pieces of one stripe), then the items of
-  // the other rows
+  // Stripe pieces go through kXcds queues: queue q holds the stripes k with
+  // k % kXcds == q, in increasing k, a unit being `per_unit` consecutive
+  // pieces of one stripe.  Workgroups are dispatched to the XCDs round-robin,
+  // so a wave of workgroup w serves queue w % kXcds first: each XCD then
+  // gathers from its own stripe, which its L2 can hold, rather than every XCD
+  // caching the same one.  A wave then takes items of the other rows from the
+  // shared queue, and finally helps drain the other XCDs' stripe queues, so
+  // every queue is emptied whatever the dispatch order.
   const uint64_t pieces = (uint64_t)uniform(ctl->n_striped) * ntiles;  // per stripe
-  // at most stripes.per_unit, and few enough that every stripe has several
-  // units per wavefront of the grid (so the waves in flight share a stripe)
+  const uint32_t K = stripes.stripes > 1 ? stripes.stripes : 0u;
   const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
-  const uint32_t per_unit =
-      (uint32_t)max(1ull, min((unsigned long long)stripes.per_unit, pieces / (4ull * waves)));
+  // at most stripes.per_unit, and few enough that every stripe has several
+  // units per wavefront of its XCD
+  const uint32_t per_unit = (uint32_t)max(
+      1ull, min((unsigned long long)stripes.per_unit, pieces * kXcds / (4ull * waves)));
   const uint64_t per_stripe = (pieces + per_unit - 1) / per_unit;
-  const uint64_t piece_units = stripes.stripes > 1 ? per_stripe * stripes.stripes : 0ull;
-  const uint64_t total = piece_units + uniform(ctl->total_units);
-  const uint32_t skip_nnz = stripes.stripes > 1 ? stripes.min_nnz : 0u;
-  for (;;) {
-    uint64_t u = 0;
-    if (lane == 0) u = atomicAdd(&ctl->next_unit, 1ull);
-    u = ((uint64_t)readlane_u32((uint32_t)(u >> 32), 0) << 32) | readlane_u32((uint32_t)u, 0);
-    if (u >= total) break;  // every wave reaches this once the queue is drained
-    const bool piece = u < piece_units;
-    uint32_t k = 0;
-    uint64_t q = 0, q_end = 1;
-    if (piece) {
-      k = (uint32_t)(u / per_stripe);
-      q = (u - k * per_stripe) * per_unit;
-      q_end = min(q + per_unit, pieces);
-    }
-    for (; q < q_end; q++) {
-      uint32_t tile, m, a, b, row;
+  const uint64_t item_units = uniform(ctl->total_units);
+  const uint32_t skip_nnz = K ? stripes.min_nnz : 0u;
+  const uint32_t home = blockIdx.x % kXcds;
+  for (uint32_t step = 0; step <= kXcds; step++) {
+    const uint32_t qid = step == 0 ? home : (step == 1 ? kXcds : (home + step - 1) % kXcds);
+    const uint32_t nk = qid < kXcds && qid < K ? (K - qid + kXcds - 1) / kXcds : 0u;
+    const uint64_t qtotal = qid == kXcds ? item_units : per_stripe * nk;
+    if (qtotal == 0) continue;
+    for (;;) {
+      uint64_t u = 0;
+      if (lane == 0) u = atomicAdd(&ctl->next[qid], 1ull);
+      u = ((uint64_t)readlane_u32((uint32_t)(u >> 32), 0) << 32) | readlane_u32((uint32_t)u, 0);
+      if (u >= qtotal) break;  // this queue is drained
+      const bool piece = qid < kXcds;
+      uint32_t k = 0;
+      uint64_t q = 0, q_end = 1;
       if (piece) {
-        const uint32_t i = (uint32_t)(q / ntiles);
-        tile = (uint32_t)(q - (uint64_t)i * ntiles);
-        const StripedRow sr = stripes.list[i];
-        m = uniform(sr.matrix);
-        row = uniform(sr.row);
-        const uint32_t s = uniform(mats[m].rows[row]);
-        const uint32_t n = uniform(mats[m].rows[row + 1]) - s;
-        a = s + (uint32_t)((uint64_t)n * k / stripes.stripes);
-        b = s + (uint32_t)((uint64_t)n * (k + 1) / stripes.stripes);
-        if (a == b) continue;
-      } else {
-        const uint64_t v = u - piece_units;
-        const uint32_t item = (uint32_t)(v / ntiles);
-        tile = (uint32_t)(v - (uint64_t)item * ntiles);
-        m = find_matrix<&MatrixDesc::item_begin>(mats, count, item);
-        const uint32_t rows0 = uniform(mats[m].rows[0]);
-        const uint32_t rend = uniform(mats[m].rows[mats[m].length]);
-        a = rows0 + (item - mats[m].item_begin) * seg;
-        b = a + min(seg, rend - a);
-        row = ~0u;
+        const uint32_t jj = (uint32_t)(u / per_stripe);
+        k = qid + jj * kXcds;
+        q = (u - (uint64_t)jj * per_stripe) * per_unit;
+        q_end = min(q + per_unit, pieces);
       }
-      const MatrixDesc M = mats[m];
-      const uint32_t k0 = tile * (kWave * V) + lane * V;
-      const uint32_t koff = (k0 < samples) ? k0 : 0u;
-      const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
-      hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row,
-                                       row == ~0u ? skip_nnz : 0u);
+      for (; q < q_end; q++) {
+        uint32_t tile, m, a, b, row;
+        if (piece) {
+          const uint32_t i = (uint32_t)(q / ntiles);
+          tile = (uint32_t)(q - (uint64_t)i * ntiles);
+          const StripedRow sr = stripes.list[i];
+          m = uniform(sr.matrix);
+          row = uniform(sr.row);
+          const uint32_t s = uniform(mats[m].rows[row]);
+          const uint32_t n = uniform(mats[m].rows[row + 1]) - s;
+          a = s + (uint32_t)((uint64_t)n * k / K);
+          b = s + (uint32_t)((uint64_t)n * (k + 1) / K);
+          if (a == b) continue;
+        } else {
+          const uint32_t item = (uint32_t)(u / ntiles);
+          tile = (uint32_t)(u - (uint64_t)item * ntiles);
+          m = find_matrix<&MatrixDesc::item_begin>(mats, count, item);
+          const uint32_t rows0 = uniform(mats[m].rows[0]);
+          const uint32_t rend = uniform(mats[m].rows[mats[m].length]);
+          a = rows0 + (item - mats[m].item_begin) * seg;
+          b = a + min(seg, rend - a);
+          row = ~0u;
+        }
+        const MatrixDesc M = mats[m];
+        const uint32_t k0 = tile * (kWave * V) + lane * V;
+        const uint32_t koff = (k0 < samples) ? k0 : 0u;
+        const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
+        hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row,
+                                         row == ~0u ? skip_nnz : 0u);
+      }
     }
   }
 }
@@ -658,8 +670,8 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
     if (tid == kBlock - 1) carry += part[kBlock - 1];
     __syncthreads();
   }
+  if (tid <= kXcds) ctl->next[tid] = 0;
   if (tid == 0) {
-    ctl->next_unit = 0;
     ctl->n_striped = 0;
     ctl->invalid = bad;
     ctl->seg = seg;
