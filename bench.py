@@ -105,6 +105,10 @@ def cpu_baseline(params, m, samples, target_s, gpu_hashes, what):
     h2, used = oracle.calc_omp(rs, ln_cs, betas, sub2.data, sub2.indices, sub2.indptr, threads)
     t2 = time.perf_counter() - t
     ok = bool(np.array_equal(h2, gpu_hashes[:n2]) and np.array_equal(h1, gpu_hashes[:n1]))
+    # information only: how often FP64 arithmetic would pick another argmin
+    n3 = max(1, min(n1, 2000))
+    h64 = oracle.calc_f64(rs, ln_cs, betas, m.data, m.indices, m.indptr[:n3 + 1])
+    flips = float(np.mean(np.any(h64 != gpu_hashes[:n3], axis=2)))
     return {
         "value": sub2.nnz * samples / t2, "unit": "nnz*samples/s", "cores": int(used),
         "kind": "port",
@@ -112,6 +116,9 @@ def cpu_baseline(params, m, samples, target_s, gpu_hashes, what):
         "single_thread_value": sub.nnz * samples / t1,
         "single_thread_sample": "%s: first %d rows (%d nnz), %.1f s" % (what, n1, sub.nnz, t1),
         "sample_bit_exact_vs_gpu": ok,
+        "fp64_argmin_flip_rate": flips,
+        "fp64_flip_sample": "%s: first %d rows, (row, sample) hashes where an FP64 evaluation "
+                            "of the same formula selects another nonzero or t" % (what, n3),
     }
 
 
