@@ -68,8 +68,11 @@ struct StripePlan {
   uint32_t stripes;    // 1: no striping
   uint32_t min_nnz;    // rows with at least this many nonzeros are striped
   uint32_t per_unit;   // pieces handed out per queue unit
+  uint32_t queues;     // stripe queues: kXcds (one per XCD) or 1
   StripedRow *list;    // capacity: every row of the batch
 };
+// Stripes up to this size get one queue per XCD (each XCD's L2 holds 4 MiB).
+constexpr uint64_t kXcdStripeBytes = 8ull << 20;
 
 // Stripe count and row threshold for a table of dim x samples parameters:
 // stripes of about `stripe_mib` MiB (at most kMaxStripes of them), pieces of

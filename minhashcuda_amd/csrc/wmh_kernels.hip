@@ -536,28 +536,31 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
   const uint32_t ntiles = uniform(ctl->ntiles);
-  // Stripe pieces go through kXcds queues: queue q holds the stripes k with
-  // k % kXcds == q, in increasing k, a unit being `per_unit` consecutive
-  // pieces of one stripe.  Workgroups are dispatched to the XCDs round-robin,
-  // so a wave of workgroup w serves queue w % kXcds first: each XCD then
-  // gathers from its own stripe, which its L2 can hold, rather than every XCD
-  // caching the same one.  A wave then takes items of the other rows from the
-  // shared queue, and finally helps drain the other XCDs' stripe queues, so
-  // every queue is emptied whatever the dispatch order.
+  // Stripe pieces go through Q = stripes.queues queues (1 or kXcds): queue q
+  // holds the stripes k with k % Q == q, in increasing k, a unit being
+  // `per_unit` consecutive pieces of one stripe.  Workgroups are dispatched to
+  // the XCDs round-robin, so with Q = kXcds a wave of workgroup w serves queue
+  // w % kXcds first: each XCD then gathers from its own stripe, which its L2
+  // can hold, rather than every XCD caching the same one (stripes larger than
+  // an L2 are better shared, Q = 1, so that together they stay in the
+  // Infinity Cache).  A wave then takes items of the other rows from the
+  // shared queue, and finally helps drain the other stripe queues, so every
+  // queue is emptied whatever the dispatch order.
   const uint64_t pieces = (uint64_t)uniform(ctl->n_striped) * ntiles;  // per stripe
   const uint32_t K = stripes.stripes > 1 ? stripes.stripes : 0u;
   const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
   // at most stripes.per_unit, and few enough that every stripe has several
   // units per wavefront of its XCD
+  const uint32_t Q = stripes.queues;
   const uint32_t per_unit = (uint32_t)max(
-      1ull, min((unsigned long long)stripes.per_unit, pieces * kXcds / (4ull * waves)));
+      1ull, min((unsigned long long)stripes.per_unit, pieces * Q / (4ull * waves)));
   const uint64_t per_stripe = (pieces + per_unit - 1) / per_unit;
   const uint64_t item_units = uniform(ctl->total_units);
   const uint32_t skip_nnz = K ? stripes.min_nnz : 0u;
-  const uint32_t home = blockIdx.x % kXcds;
-  for (uint32_t step = 0; step <= kXcds; step++) {
-    const uint32_t qid = step == 0 ? home : (step == 1 ? kXcds : (home + step - 1) % kXcds);
-    const uint32_t nk = qid < kXcds && qid < K ? (K - qid + kXcds - 1) / kXcds : 0u;
+  const uint32_t home = blockIdx.x % Q;
+  for (uint32_t step = 0; step <= Q; step++) {
+    const uint32_t qid = step == 0 ? home : (step == 1 ? kXcds : (home + step - 1) % Q);
+    const uint32_t nk = qid < Q && qid < K ? (K - qid + Q - 1) / Q : 0u;
     const uint64_t qtotal = qid == kXcds ? item_units : per_stripe * nk;
     if (qtotal == 0) continue;
     for (;;) {
@@ -570,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
       uint64_t q = 0, q_end = 1;
       if (piece) {
         const uint32_t jj = (uint32_t)(u / per_stripe);
-        k = qid + jj * kXcds;
+        k = qid + jj * Q;
         q = (u - (uint64_t)jj * per_stripe) * per_unit;
         q_end = min(q + per_unit, pieces);
       }
@@ -946,8 +949,11 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
-  StripePlan stripes{1u, 0xffffffffu, 1u, striped};
+  StripePlan stripes{1u, 0xffffffffu, 1u, 1u, striped};
   stripe_shape(dim, samples, tuning.stripe_mib, &stripes.stripes, &stripes.min_nnz);
+  // stripes that fit an XCD's L2 (twice over) get one queue per XCD
+  const uint64_t stripe_bytes = (uint64_t)dim * 12u * sp / stripes.stripes;
+  stripes.queues = stripe_bytes <= kXcdStripeBytes ? kXcds : 1u;
   // about half an item's nonzeros of the shortest pieces per queue unit
   stripes.per_unit = std::max(1u, (uint32_t)tuning.seg / (2u * kStripePieceNnz));
   if (stripes.stripes > 1 && striped == nullptr) return hipErrorInvalidValue;
