@@ -244,3 +244,46 @@ def test_stripes_edge_values(libmhcuda, capi):
         got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
         assert np.array_equal(got, ref), seg
     libmhcuda.minhash_cuda_fini(gen)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batches_and_tunings(libmhcuda, capi, seed):
+    """Seeded random batches (row lengths from empty to several thousand,
+    sorted or shuffled columns, some columns >= dim, zero weights) under
+    random tunings, including column stripes: every hash equals the oracle's."""
+    rng = np.random.default_rng(1000 + seed)
+    dim = int(rng.choice([700, 3000, 20000]))
+    samples = int(rng.choice([2, 64, 100, 128, 256]))
+    count = int(rng.integers(1, 6))
+    mats = []
+    for i in range(count):
+        rows = int(rng.integers(0, 300))
+        kind = rng.integers(0, 4, rows)
+        nnz = np.where(kind == 0, 0, np.where(kind == 1, rng.integers(1, 40, rows),
+                       np.where(kind == 2, rng.integers(40, 600, rows),
+                                rng.integers(600, 4000, rows))))
+        nnz = np.minimum(nnz, dim + 5)
+        m = workloads.batched_csr(nnz.astype(np.int64), dim=dim + 5, sigma=1.5,
+                                  seed=seed * 10 + i)
+        if rng.random() < 0.5:
+            m = shuffled_rows(m, seed * 10 + i)
+        m.data[rng.random(m.nnz) < 0.03] = 0.0
+        mats.append(m)
+    spl = int(rng.choice([1, 2, 4]))
+    seg = int(rng.choice([64, 256, 1024, 4096]))
+    wpc = int(rng.choice([1, 8, 16, 32]))
+    stripe = int(rng.choice([0, 1, 2]))
+    say("seed", seed, "dim", dim, "samples", samples, "count", count, "spl", spl, "seg", seg,
+        "wpc", wpc, "stripe MiB", stripe)
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=seed)
+    params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+    g = ctypes.c_void_p(gen)
+    assert capi.mhcuda_set_tuning(g, spl, seg, wpc) == 0
+    assert capi.mhcuda_set_stripes(g, stripe) == 0
+    try:
+        got = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+    for i, (h, m) in enumerate(zip(got, mats)):
+        ref = oracle.calc(*params, m.data, m.indices.astype(np.uint32), m.indptr)
+        assert np.array_equal(h, ref), (seed, i)
