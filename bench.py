@@ -47,6 +47,8 @@ def parse():
                    help="serve: comma-separated arrival rates (requests/s)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
                    help="serve: duration of each arrival-rate run")
+    p.add_argument("--slots", type=int, default=4,
+                   help="serve: launches in flight at once (one stream and worker each)")
     p.add_argument("--pinned", action="store_true",
                    help="serve: requests live in pinned host memory (direct DMA)")
     p.add_argument("--spl", type=int, default=0, help="samples per lane (0: default)")
@@ -254,7 +256,7 @@ def serve_bench(args, torch):
             arrs.append((ot, ot.numpy().view(np.uint32)))
             pinned_pool.append(arrs)
     gen = libMHCUDA.minhash_cuda_init(dim, samples, seed=1, devices=1)
-    server = serving.Server(gen, memory_budget=4 << 30, slots=3, max_batch_requests=256)
+    server = serving.Server(gen, memory_budget=4 << 30, slots=args.slots, max_batch_requests=256)
     # warm-up: every buffer reaches its working size
     for t in [server.submit(m) for m in pool[:64]]:
         server.wait(t)
@@ -318,9 +320,9 @@ def serve_bench(args, torch):
         "config": {"workload": "serving: Poisson arrivals, requests of 1-4096 rows "
                                "(log-uniform), dim 2^22, nnz/row lognormal(median 200, sigma 1) "
                                "clipped to 50000, lognormal(0,1) weights, 256 samples, "
-                               "parameters resident (12.9 GB), 3 slots, 4 GiB budget, "
+                               "parameters resident (12.9 GB), %d slots, 4 GiB budget, "
                                "request arrays in %s host memory"
-                               % ("pinned" if args.pinned else "pageable"),
+                               % (args.slots, "pinned" if args.pinned else "pageable"),
                    "pool_requests": len(pool)},
         "sweep": results}), flush=True)
 

@@ -1,6 +1,6 @@
 """Python client of the engine's serving loop (mhcuda_serve_* in minhashcuda.h).
 
-    server = Server(gen, memory_budget=1 << 30, slots=3)
+    server = Server(gen, memory_budget=1 << 30, slots=4)
     ticket = server.submit(csr)          # scipy csr_matrix, float32 data
     hashes = server.wait(ticket)         # (rows, samples, 2) uint32
     print(server.stats())
@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libMHCUDA.s
 
 
 class Server:
-    def __init__(self, gen, memory_budget=1 << 30, slots=3, max_batch_requests=256, lib=None):
+    def __init__(self, gen, memory_budget=1 << 30, slots=4, max_batch_requests=256, lib=None):
         self.lib = lib or capi_mod.load(LIB_PATH)
         cfg = capi_mod.ServeConfig(memory_budget, slots, max_batch_requests)
         status = ctypes.c_int(0)
