@@ -413,13 +413,16 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
 // Hashes CSR positions [a, b) of matrix M (one item or one stripe piece) for
 // the lane's samples.  In a split row the argmin tracks the CSR position
 // instead of the column.  `row` is the row containing a, or ~0u to search
-// for it.  Unless skip_nnz is 0, rows with at least `skip_nnz` nonzeros are
-// skipped (their stripe pieces hash them); skip_nnz >= 64.
+// for it; for a stripe piece (`piece_end` != 0) [a, b) lies inside `row`,
+// whose positions are [piece_start, piece_end), and no row bound is read.
+// Unless skip_nnz is 0, rows with at least `skip_nnz` nonzeros are skipped
+// (their stripe pieces hash them); skip_nnz >= 64.
 template <int V, int U>
 __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint32_t dim,
                                           uint32_t sp, uint32_t samples, uint32_t k0,
                                           LaneOffsets lo, const MatrixDesc &M,
                                           uint32_t a, uint32_t b, uint32_t row,
+                                          uint32_t piece_start, uint32_t piece_end,
                                           uint32_t skip_nnz) {
   static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
   const uint32_t lane = lane_id();
@@ -428,13 +431,15 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   const uint32_t *__restrict__ rows = M.rows;
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
+  const bool in_row = piece_end != 0u;
   uint32_t r = row != ~0u ? row : find_row(rows, M.length, a);
-  uint32_t row_start = uniform(rows[r]);
+  uint32_t row_start = in_row ? piece_start : uniform(rows[r]);
   // rows[rp_base + lane]: the next 64 row ends, read with one vector load
+  // (never needed inside one row: no position of [a, b) ends it before b)
   uint32_t rp_base = r + 1;
-  uint32_t rp = (rp_base + lane <= M.length) ? rows[rp_base + lane] : 0xffffffffu;
+  uint32_t rp = (!in_row && rp_base + lane <= M.length) ? rows[rp_base + lane] : 0xffffffffu;
   uint32_t rp_idx = 0;
-  uint32_t row_end = readlane_u32(rp, 0);
+  uint32_t row_end = in_row ? piece_end : readlane_u32(rp, 0);
   bool split = row_start < a || row_end > b;
   bool skip = skip_nnz != 0u && row_end - row_start >= skip_nnz;
   Argmin<V> acc;
@@ -570,25 +575,36 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
       if (u >= qtotal) break;  // this queue is drained
       const bool piece = qid < kXcds;
       uint32_t k = 0;
-      uint64_t q = 0, q_end = 1;
+      uint64_t q0 = 0, q_end = 1;
+      // lane l holds the row of the unit's piece l (one vector load of the
+      // list and of the row bounds for the whole unit)
+      uint32_t lm = 0, lrow = 0, ls = 0, le = 0;
       if (piece) {
         const uint32_t jj = (uint32_t)(u / per_stripe);
         k = qid + jj * Q;
-        q = (u - (uint64_t)jj * per_stripe) * per_unit;
-        q_end = min(q + per_unit, pieces);
+        q0 = (u - (uint64_t)jj * per_stripe) * per_unit;
+        q_end = min(q0 + per_unit, pieces);
+        if (q0 + lane < q_end) {
+          const StripedRow sr = stripes.list[(uint32_t)((q0 + lane) / ntiles)];
+          lm = sr.matrix;
+          lrow = sr.row;
+          const uint32_t *rws = mats[lm].rows;
+          ls = rws[lrow];
+          le = rws[lrow + 1];
+        }
       }
-      for (; q < q_end; q++) {
-        uint32_t tile, m, a, b, row;
+      for (uint64_t q = q0; q < q_end; q++) {
+        uint32_t tile, m, a, b, row, ps = 0, pe = 0;
         if (piece) {
-          const uint32_t i = (uint32_t)(q / ntiles);
-          tile = (uint32_t)(q - (uint64_t)i * ntiles);
-          const StripedRow sr = stripes.list[i];
-          m = uniform(sr.matrix);
-          row = uniform(sr.row);
-          const uint32_t s = uniform(mats[m].rows[row]);
-          const uint32_t n = uniform(mats[m].rows[row + 1]) - s;
-          a = s + (uint32_t)((uint64_t)n * k / K);
-          b = s + (uint32_t)((uint64_t)n * (k + 1) / K);
+          const uint32_t l = (uint32_t)(q - q0);
+          tile = (uint32_t)(q % ntiles);
+          m = readlane_u32(lm, l);
+          row = readlane_u32(lrow, l);
+          ps = readlane_u32(ls, l);
+          pe = readlane_u32(le, l);
+          const uint32_t n = pe - ps;
+          a = ps + (uint32_t)((uint64_t)n * k / K);
+          b = ps + (uint32_t)((uint64_t)n * (k + 1) / K);
           if (a == b) continue;
         } else {
           const uint32_t item = (uint32_t)(u / ntiles);
@@ -604,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
         const uint32_t k0 = tile * (kWave * V) + lane * V;
         const uint32_t koff = (k0 < samples) ? k0 : 0u;
         const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
-        hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row,
+        hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row, ps, pe,
                                          row == ~0u ? skip_nnz : 0u);
       }
     }
@@ -954,8 +970,10 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   // stripes that fit an XCD's L2 (twice over) get one queue per XCD
   const uint64_t stripe_bytes = (uint64_t)dim * 12u * sp / stripes.stripes;
   stripes.queues = stripe_bytes <= kXcdStripeBytes ? kXcds : 1u;
-  // about half an item's nonzeros of the shortest pieces per queue unit
-  stripes.per_unit = std::max(1u, (uint32_t)tuning.seg / (2u * kStripePieceNnz));
+  // about half an item's nonzeros of the shortest pieces per queue unit (at
+  // most one per lane: a unit's pieces are read with one vector load)
+  stripes.per_unit =
+      std::min<uint32_t>(kWave, std::max(1u, (uint32_t)tuning.seg / (2u * kStripePieceNnz)));
   if (stripes.stripes > 1 && striped == nullptr) return hipErrorInvalidValue;
   const uint32_t seg = (uint32_t)tuning.seg;
   const uint32_t target = auto_seg ? (uint32_t)std::max(1, num_cus * tuning.waves_per_cu * 4) : 0u;
