@@ -231,11 +231,15 @@ def test_python_input_checks(L):
     with pytest.raises(ValueError):
         L.minhash_cuda_assign_vars(gen, np.zeros(5, np.float32), np.zeros(160, np.float32),
                                    np.zeros(160, np.float32))
-    # int64 indices / indptr are force-cast (python.cc:317,323)
+    # int64 indices / indptr are force-cast (python.cc:317,323): same hashes
     m = sp.csr_matrix(dense)
-    m.indices = m.indices.astype(np.int64)
-    m.indptr = m.indptr.astype(np.int64)
-    assert L.minhash_cuda_calc(gen, m).shape == (10, 16, 2)
+    want = L.minhash_cuda_calc(gen, m)
+    m64 = sp.csr_matrix(dense)
+    m64.indices = m64.indices.astype(np.int64)
+    m64.indptr = m64.indptr.astype(np.int64)
+    assert np.array_equal(L.minhash_cuda_calc(gen, m64), want)
+    # row_finish past the end is clamped (python.cc:341)
+    assert np.array_equal(L.minhash_cuda_calc(gen, m, row_start=3, row_finish=1000), want[3:])
     L.minhash_cuda_fini(gen)
 
 
