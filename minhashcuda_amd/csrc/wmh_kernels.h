@@ -52,14 +52,18 @@ struct Tuning {
 };
 
 // Column stripes of the persistent launch.  When the parameter table is
-// larger than one stripe, every row with at least `min_nnz` nonzeros is cut
-// into `stripes` pieces of equal nonzero count; the work queue hands out all
-// first pieces, then all second pieces, and so on.  For rows whose columns
-// are sorted and spread over the table, piece k gathers from the k-th
-// `1 / stripes` of the table, so the waves in flight share a stripe of
-// parameters small enough to stay in the caches and address translation
-// caches.  Any row order is still hashed exactly: the pieces are merged like
-// rows split across work items.  Other rows are packed into items as before.
+// larger than one stripe, the columns are cut into `stripes` ranges of
+// `width` columns, and every row with at least `min_nnz` nonzeros is cut into
+// one piece per range: piece k holds the row's positions from the first
+// whose column is >= k * width to the first whose column is >= (k + 1) *
+// width (found by binary search; the last piece runs to the row's end).  The
+// work queue hands out all first pieces, then all second pieces, and so on,
+// so for rows whose columns are sorted the waves in flight gather from one
+// stripe of parameters, small enough to stay in the caches and address
+// translation caches.  A binary search is monotone in its key whatever the
+// column order, so the pieces of any row partition it exactly and are merged
+// like rows split across work items.  Other rows are packed into items as
+// before.
 struct StripedRow {
   uint32_t matrix;
   uint32_t row;
@@ -69,14 +73,15 @@ struct StripePlan {
   uint32_t min_nnz;    // rows with at least this many nonzeros are striped
   uint32_t per_unit;   // pieces handed out per queue unit
   uint32_t queues;     // stripe queues: kXcds (one per XCD) or 1
+  uint32_t width;      // columns per stripe: ceil(dim / stripes)
   StripedRow *list;    // capacity: every row of the batch
 };
 // Stripes up to this size get one queue per XCD (each XCD's L2 holds 4 MiB).
 constexpr uint64_t kXcdStripeBytes = 8ull << 20;
 
 // Stripe count and row threshold for a table of dim x samples parameters:
-// stripes of about `stripe_mib` MiB (at most kMaxStripes of them), pieces of
-// at least kStripePieceNnz nonzeros, rows of at least 1024.
+// stripes of about `stripe_mib` MiB (at most kMaxStripes of them), rows of at
+// least 1024 nonzeros and of kStripePieceNnz nonzeros per stripe.
 constexpr uint32_t kMaxStripes = 256;
 constexpr int kDefaultStripeMib = 4;
 constexpr uint32_t kStripePieceNnz = 32;

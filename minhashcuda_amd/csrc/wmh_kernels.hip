@@ -440,7 +440,9 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   uint32_t rp = (!in_row && rp_base + lane <= M.length) ? rows[rp_base + lane] : 0xffffffffu;
   uint32_t rp_idx = 0;
   uint32_t row_end = in_row ? piece_end : readlane_u32(rp, 0);
-  bool split = row_start < a || row_end > b;
+  // a stripe piece always merges through the keyed minima, even when it
+  // holds the whole row (the fix-up pass reads every striped row as keys)
+  bool split = in_row || row_start < a || row_end > b;
   bool skip = skip_nnz != 0u && row_end - row_start >= skip_nnz;
   Argmin<V> acc;
   acc.init();
@@ -518,6 +520,37 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   if (row_end > b && !skip) flush_row<V>(M.output, r, samples, k0, true, acc);  // continues later
 }
 
+// Stripe piece k (of K) of the CSR positions [s, e): from the first position
+// whose column is >= k * width to the first whose column is >= (k + 1) *
+// width, or to e for the last stripe.  The two binary searches run together.
+// Whatever the column order, a search's result never decreases as its key
+// grows (two keys take the same path until the first probed column lying
+// between them, where the smaller key turns left and the larger right), so
+// the end of piece k is the start of piece k + 1 and the K pieces partition
+// [s, e).
+__device__ __forceinline__ void stripe_piece(const uint32_t *__restrict__ cols, uint32_t s,
+                                             uint32_t e, uint32_t k, uint32_t K,
+                                             uint32_t width, uint32_t &a, uint32_t &b) {
+  const uint64_t x0 = (uint64_t)k * width, x1 = x0 + width;
+  uint32_t lo0 = s, hi0 = k == 0 ? s : e;
+  uint32_t lo1 = k + 1 == K ? e : s, hi1 = e;
+  while (lo0 < hi0 || lo1 < hi1) {
+    const uint32_t m0 = lo0 + ((hi0 - lo0) >> 1), m1 = lo1 + ((hi1 - lo1) >> 1);
+    const uint32_t c0 = lo0 < hi0 ? cols[m0] : 0u;
+    const uint32_t c1 = lo1 < hi1 ? cols[m1] : 0u;
+    if (lo0 < hi0) {
+      if (c0 < x0) lo0 = m0 + 1;
+      else hi0 = m0;
+    }
+    if (lo1 < hi1) {
+      if (c1 < x1) lo1 = m1 + 1;
+      else hi1 = m1;
+    }
+  }
+  a = lo0;
+  b = lo1;
+}
+
 // Matrix whose range of `field` values contains x: the largest m with
 // mats[m].field <= x (mats sorted by field).
 template <uint32_t MatrixDesc::*Field>
@@ -576,9 +609,10 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
       const bool piece = qid < kXcds;
       uint32_t k = 0;
       uint64_t q0 = 0, q_end = 1;
-      // lane l holds the row of the unit's piece l (one vector load of the
-      // list and of the row bounds for the whole unit)
-      uint32_t lm = 0, lrow = 0, ls = 0, le = 0;
+      // lane l holds the row of the unit's piece l and the piece's bounds
+      // (one vector load of the list and of the row bounds for the whole
+      // unit, the lanes' binary searches side by side)
+      uint32_t lm = 0, lrow = 0, ls = 0, le = 0, la = 0, lb = 0;
       if (piece) {
         const uint32_t jj = (uint32_t)(u / per_stripe);
         k = qid + jj * Q;
@@ -591,6 +625,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
           const uint32_t *rws = mats[lm].rows;
           ls = rws[lrow];
           le = rws[lrow + 1];
+          stripe_piece(mats[lm].cols - mats[lm].base, ls, le, k, K, stripes.width, la, lb);
         }
       }
       for (uint64_t q = q0; q < q_end; q++) {
@@ -602,9 +637,8 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
           row = readlane_u32(lrow, l);
           ps = readlane_u32(ls, l);
           pe = readlane_u32(le, l);
-          const uint32_t n = pe - ps;
-          a = ps + (uint32_t)((uint64_t)n * k / K);
-          b = ps + (uint32_t)((uint64_t)n * (k + 1) / K);
+          a = readlane_u32(la, l);
+          b = readlane_u32(lb, l);
           if (a == b) continue;
         } else {
           const uint32_t item = (uint32_t)(u / ntiles);
@@ -965,8 +999,9 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
-  StripePlan stripes{1u, 0xffffffffu, 1u, 1u, striped};
+  StripePlan stripes{1u, 0xffffffffu, 1u, 1u, 0u, striped};
   stripe_shape(dim, samples, tuning.stripe_mib, &stripes.stripes, &stripes.min_nnz);
+  stripes.width = (uint32_t)(((uint64_t)dim + stripes.stripes - 1) / stripes.stripes);
   // stripes that fit an XCD's L2 (twice over) get one queue per XCD
   const uint64_t stripe_bytes = (uint64_t)dim * 12u * sp / stripes.stripes;
   stripes.queues = stripe_bytes <= kXcdStripeBytes ? kXcds : 1u;

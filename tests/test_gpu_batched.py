@@ -182,24 +182,27 @@ def test_pipelined_host_path_many_pieces(libmhcuda, batch, expected, monkeypatch
 def test_stripes_edge_values(libmhcuda, capi):
     """Striped rows holding the edge cases: zero, negative, NaN, infinite and
     subnormal weights, columns >= dim, a row of exactly the striping threshold,
-    an all-zero striped row, and a tie between two columns with identical
-    parameters whose positions fall in different stripe pieces (the earlier
+    an all-zero striped row, rows whose columns all fall in one stripe (the
+    first, a middle one, only columns >= dim), unsorted rows whose pieces are
+    cut by binary search over unordered columns, and a tie between two columns
+    with identical parameters on either side of a stripe boundary (the earlier
     position must win, as in the sequential walk)."""
     dim, samples = 4000, 64
     rng = np.random.default_rng(17)
     rs = rng.gamma(2.0, 1.0, (samples, dim)).astype(np.float32)
     ln_cs = np.log(rng.gamma(2.0, 1.0, (samples, dim))).astype(np.float32)
     betas = rng.uniform(0, 1, (samples, dim)).astype(np.float32)
-    j1, j2 = 1500, 1501
+    width = -(-dim // 3)  # 3 stripes of 1334 columns
+    j1, j2 = width - 1, width
     for t in (rs, ln_cs, betas):
         t[:, j2] = t[:, j1]
 
-    def row_sorted(n, hi=dim):
-        return np.sort(rng.choice(hi, n, replace=False)).astype(np.uint32)
+    def row_sorted(n, hi=dim, lo=0):
+        return np.sort(rng.choice(np.arange(lo, hi), n, replace=False)).astype(np.uint32)
 
     rows_cols, rows_w = [], []
-    # A: 3000 nonzeros, stripe pieces of 1000; the tied pair sits at positions 999 | 1000,
-    # and the last columns are >= dim
+    # A: 3000 nonzeros; the tied pair sits at positions 999 | 1000, the last column of
+    # stripe 0 and the first of stripe 1, and the last columns are >= dim
     below = np.sort(rng.choice(j1, 999, replace=False))
     above = np.sort(rng.choice(np.arange(j2 + 1, dim + 10), 1999, replace=False))
     c = np.concatenate([below, [j1, j2], above]).astype(np.uint32)
@@ -223,6 +226,15 @@ def test_stripes_edge_values(libmhcuda, capi):
     rows_cols.append(row_sorted(1023)); rows_w.append(rng.lognormal(0, 1, 1023).astype(np.float32))
     # D: striped, every weight zero
     rows_cols.append(row_sorted(2000)); rows_w.append(np.zeros(2000, np.float32))
+    # F, G, H: every column in stripe 0, in stripe 1, or >= dim (one piece holds the row)
+    for lo, hi in ((0, width), (width, 2 * width), (dim, dim + 10)):
+        n = min(1100, hi - lo) if hi - lo >= 1024 else 0
+        c = row_sorted(n, hi, lo) if n else np.repeat(np.arange(lo, hi, dtype=np.uint32), 120)[:1200]
+        rows_cols.append(c); rows_w.append(rng.lognormal(0, 1, len(c)).astype(np.float32))
+    # I: unsorted, columns alternating between the two ends of the table
+    c = row_sorted(2400)
+    c = np.stack([c[:1200], c[1200:][::-1]], 1).ravel().astype(np.uint32)
+    rows_cols.append(c); rows_w.append(rng.lognormal(0, 1, 2400).astype(np.float32))
     for n in (1, 5, 60):
         rows_cols.append(row_sorted(n)); rows_w.append(rng.lognormal(0, 1, n).astype(np.float32))
     indptr = np.concatenate([[0], np.cumsum([len(c) for c in rows_cols])]).astype(np.uint32)
