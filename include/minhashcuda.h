@@ -140,10 +140,12 @@ MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen,
                                int *waves_per_cu);
 
 /* Column stripes of the persistent kernel: with a parameter table larger
- * than `stripe_mib` MiB, rows of many nonzeros are cut into as many pieces as
- * the table has stripes of that size, and all first pieces are hashed before
- * all second pieces, and so on, so that concurrent gathers share one stripe
- * of the table (for rows with sorted, spread columns).  0 turns stripes off.
+ * than `stripe_mib` MiB, the columns are cut into as many ranges as the table
+ * has stripes of that size, rows of many nonzeros are cut into one piece per
+ * column range, and all first pieces are hashed before all second pieces, and
+ * so on, so that concurrent gathers share one stripe of the table (for rows
+ * with sorted columns; any column order is hashed exactly).  0 turns stripes
+ * off.
  * The hashes do not depend on it. */
 MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib);
 /* The stripe size, the resulting stripe count for this generator's table

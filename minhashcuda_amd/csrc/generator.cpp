@@ -202,7 +202,8 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
                           (uint32_t)chunks, (BatchCtl *)(dev + ctl_off),
                           (StripedRow *)(dev + layout.list_off), gen->tuning, gen->seg_auto,
-                          gen->num_cus, s, gen->timing && !capturing ? gen->ev_begin : nullptr,
+                          gen->num_cus, gen->fast_div, s,
+                          gen->timing && !capturing ? gen->ev_begin : nullptr,
                           gen->timing && !capturing ? gen->ev_end : nullptr),
         mhcudaRuntimeError);
   gen->timed = gen->timing;
@@ -234,11 +235,27 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
   if (gen->timing && !capturing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
   for (const MatrixDesc &m : mats) {
     if (m.length == 0) continue;
-    HIPCH(launch_plain(gen->params, gen->dim, gen->samples, m, gen->tuning.spl, s),
+    HIPCH(launch_plain(gen->params, gen->dim, gen->samples, m, gen->tuning.spl, gen->fast_div, s),
           mhcudaRuntimeError);
   }
   if (gen->timing && !capturing) HIPCH(hipEventRecord(gen->ev_end, s), mhcudaRuntimeError);
   gen->timed = gen->timing;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen) {
+  int verbosity = gen->verbosity;
+  gen->fast_div = false;
+  HIPCH(gen->range_flag.reserve(sizeof(uint32_t)), mhcudaMemoryAllocationFailure);
+  HIPCH(hipMemsetAsync(gen->range_flag.ptr, 0, sizeof(uint32_t), gen->stream), mhcudaRuntimeError);
+  HIPCH(launch_params_range(gen->params, gen->dim, gen->samples, (uint32_t *)gen->range_flag.ptr,
+                            gen->stream),
+        mhcudaRuntimeError);
+  uint32_t bad = 1;
+  HIPCH(hipMemcpyAsync(&bad, gen->range_flag.ptr, sizeof(bad), hipMemcpyDeviceToHost, gen->stream),
+        mhcudaMemoryCopyError);
+  HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
+  gen->fast_div = bad == 0;
   return mhcudaSuccess;
 }
 
@@ -258,7 +275,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
                     gc.tuning.spl == gen->tuning.spl && gc.tuning.seg == gen->tuning.seg &&
                     gc.tuning.waves_per_cu == gen->tuning.waves_per_cu &&
                     gc.tuning.stripe_mib == gen->tuning.stripe_mib &&
-                    gc.key.size() == mats.size() &&
+                    gc.fast_div == gen->fast_div && gc.key.size() == mats.size() &&
                     std::memcmp(gc.key.data(), mats.data(), sizeof(MatrixDesc) * mats.size()) == 0;
   if (!same) {
     if (!gc.capture)
@@ -292,6 +309,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
     gc.key.swap(key);
     gc.kernel = kernel;
     gc.tuning = gen->tuning;
+    gc.fast_div = gen->fast_div;
     gc.captures++;
   }
   // timing events stay outside the graph: with graphs the timed span is the
@@ -383,6 +401,8 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
       FAIL(mhcudaRuntimeError);
     gen->params_source = MinhashCudaGenerator::kParamsSeeded;
     gen->seed = seed;
+    const MHCUDAResult rr = wmh::refresh_division_mode(gen.get());
+    if (rr != mhcudaSuccess) FAIL(rr);
   }
   if (hipStreamSynchronize(gen->stream) != hipSuccess) FAIL(mhcudaRuntimeError);
   if (verbosity > 1 && print_memory_stats(dev) != mhcudaSuccess) FAIL(mhcudaRuntimeError);
@@ -439,6 +459,7 @@ MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *cgen, const f
         mhcudaMemoryAllocationFailure);
   const float *src[3] = {rs, betas, ln_cs};
   gen->params_source = MinhashCudaGenerator::kParamsNone;  // until every table is in
+  gen->fast_div = false;                                   // until the new r are checked
   for (uint32_t which = 0; which < 3; which++) {
     for (uint32_t j0 = 0; j0 < gen->dim; j0 += chunk) {
       const uint32_t n = std::min(chunk, gen->dim - j0);
@@ -454,7 +475,7 @@ MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *cgen, const f
     }
   }
   gen->params_source = MinhashCudaGenerator::kParamsTables;
-  return mhcudaSuccess;
+  return wmh::refresh_division_mode(gen);
 }
 
 MHCUDAResult mhcuda_calc(const MinhashCudaGenerator *cgen, const float *weights,

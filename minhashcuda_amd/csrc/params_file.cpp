@@ -310,6 +310,7 @@ MHCUDAResult mhcuda_load_random_vars(const MinhashCudaGenerator *cgen, const cha
   if (h.kind == kKindSeeded && h.recipe != kRecipe) return mhcudaInvalidArguments;
   if (hipSetDevice(gen->device) != hipSuccess) return mhcudaNoSuchDevice;
   gen->params_source = MinhashCudaGenerator::kParamsNone;  // until every table is in
+  gen->fast_div = false;                                   // until the new r are checked
 
   if (h.kind == kKindSeeded) {
     if (wmh::launch_gen_params(gen->params, dim, samples, h.seed, gen->stream) != hipSuccess ||
@@ -317,7 +318,7 @@ MHCUDAResult mhcuda_load_random_vars(const MinhashCudaGenerator *cgen, const cha
       return mhcudaRuntimeError;
     gen->params_source = MinhashCudaGenerator::kParamsSeeded;
     gen->seed = h.seed;
-    return mhcudaSuccess;
+    return wmh::refresh_division_mode(gen);
   }
 
   // stream the tables in: the read of chunk i overlaps DMA + relayout of chunk i-1
@@ -375,10 +376,11 @@ MHCUDAResult mhcuda_load_random_vars(const MinhashCudaGenerator *cgen, const cha
     // the tables are partly overwritten: leave them as a deferred init does
     (void)hipMemsetAsync(gen->params, 0, (size_t)dim * 3u * gen->sp * sizeof(float), gen->stream);
     (void)hipStreamSynchronize(gen->stream);
+    gen->fast_div = false;
     return r;
   }
   gen->params_source = MinhashCudaGenerator::kParamsTables;
-  return mhcudaSuccess;
+  return wmh::refresh_division_mode(gen);
 }
 
 }  // extern "C"

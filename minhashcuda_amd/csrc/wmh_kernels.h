@@ -84,7 +84,7 @@ constexpr uint64_t kXcdStripeBytes = 8ull << 20;
 // least 1024 nonzeros and of kStripePieceNnz nonzeros per stripe.
 constexpr uint32_t kMaxStripes = 256;
 constexpr int kDefaultStripeMib = 4;
-constexpr uint32_t kStripePieceNnz = 32;
+constexpr uint32_t kStripePieceNnz = 16;
 inline void stripe_shape(uint32_t dim, uint32_t samples, int stripe_mib, uint32_t *stripes,
                          uint32_t *min_nnz) {
   const uint64_t table = (uint64_t)dim * 12u * padded_samples(samples);
@@ -99,6 +99,12 @@ inline void stripe_shape(uint32_t dim, uint32_t samples, int stripe_mib, uint32_
 hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
                              uint32_t seed, hipStream_t stream);
 
+// Sets *bad (device memory, zeroed by the caller) to nonzero unless every r of
+// the table (the first `samples` entries of each column's r run) lies in
+// [2^-60, 2^60]; NaNs count as outside.
+hipError_t launch_params_range(const float *params, uint32_t dim, uint32_t samples,
+                               uint32_t *bad, hipStream_t stream);
+
 // Scatter one [samples][ncols] FP32 table (rows of pitch ncols) into
 // params[(j0 + jj) * 3 * sp + which * sp + k].
 hipError_t launch_scatter_params(const float *staging, uint32_t ncols,
@@ -111,8 +117,10 @@ hipError_t launch_gather_params(const float *params, uint32_t j0, uint32_t ncols
 
 // One launch hashing every row of one matrix: one wavefront per
 // (row, sample tile).
+// `fast_div`: every r of the table lies in [2^-60, 2^60] (launch_params_range),
+// so the kernels may divide with the short correctly rounded sequence.
 hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
-                        const MatrixDesc &m, int spl, hipStream_t stream);
+                        const MatrixDesc &m, int spl, bool fast_div, hipStream_t stream);
 
 // Batched hashing of `count` matrices in one persistent launch (plus the
 // plan, init and fix-up passes).  `mats` and `ctl` are device memory; the
@@ -128,7 +136,7 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
-                             bool auto_seg, int num_cus, hipStream_t stream,
+                             bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
                              hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 // Weighted-Jaccard estimates of row pairs (pairs[2i], pairs[2i+1] index the

@@ -149,6 +149,31 @@ __global__ __launch_bounds__(kBlock) void gather_params_kernel(
   }
 }
 
+// Range check of the r table for the short division (div_rn_fast): *bad
+// becomes nonzero when some r of a real sample is outside [2^-60, 2^60] or
+// NaN.  One ballot and at most one atomic per wavefront.
+__global__ __launch_bounds__(kBlock) void params_range_kernel(const float *__restrict__ params,
+                                                              uint32_t dim, uint32_t samples,
+                                                              uint32_t sp,
+                                                              uint32_t *__restrict__ bad) {
+  const uint64_t total = (uint64_t)dim * samples;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  // every lane of a wavefront runs the same number of iterations, so the
+  // ballot is taken with the whole wavefront present
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(uint64_t)(kWave - 1));
+       base < total; base += stride) {
+    const uint64_t idx = base + lane_id();
+    bool out = false;
+    if (idx < total) {
+      const uint32_t j = (uint32_t)(idx / samples);
+      const uint32_t k = (uint32_t)(idx - (uint64_t)j * samples);
+      const float r = params[(uint64_t)j * 3u * sp + k];
+      out = !(r >= 0x1p-60f && r <= 0x1p60f);
+    }
+    if (__ballot(out) != 0ull && lane_id() == 0) atomicOr(bad, 1u);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The hashing core.
 
@@ -175,35 +200,50 @@ struct Argmin {
 #define WMH_GROUP_NNZ 16
 #endif
 
-// Loads V consecutive floats starting at p (8/16-byte aligned for V = 2/4).
-template <int V>
-__device__ __forceinline__ void load_v(const float *__restrict__ p, float (&out)[V]) {
-  if constexpr (V == 1) {
-    out[0] = p[0];
-  } else if constexpr (V == 2) {
-    const float2 x = *reinterpret_cast<const float2 *>(p);
-    out[0] = x.x;
-    out[1] = x.y;
-  } else {
-    const float4 x = *reinterpret_cast<const float4 *>(p);
-    out[0] = x.x;
-    out[1] = x.y;
-    out[2] = x.z;
-    out[3] = x.w;
-  }
+// a / b rounded to nearest, in 6 instructions instead of the 11 of the
+// general IEEE sequence (which also scales operands near the ends of the
+// exponent range and fixes up zeros, infinities and NaNs).  Valid for every
+// b in [2^-60, 2^60] and every finite a with |a| in [2^-25, 2^7] or a = +0,
+// which covers a = ln(w) for every positive finite float w:
+//   y = one Newton step from v_rcp_f32 is RN(1/b) for every such b (checked
+//       exhaustively on gfx950 over all 1,015,021,568 floats in
+//       [2^-60, 2^61));
+//   q0 = RN(a * y) is within 1 ulp of a / b, so rem = a - b * q0 is exact
+//       (no underflow in this range) and RN(q0 + rem * y) = RN(a / b)
+//       (P. Markstein, IBM J. Res. Dev. 34(1), 1990; J.-M. Muller et al.,
+//       Handbook of Floating-Point Arithmetic, on FMA-based division);
+//       also checked exhaustively on gfx950 against IEEE division for all
+//       2^46 significand pairs at two exponent pairs.
+// For a = -inf or NaN both give a value (-inf / NaN vs NaN) that is never
+// selected; a = +inf must take the IEEE path (+inf vs NaN).
+__device__ __forceinline__ float div_rn_fast(float a, float b) {
+  const float y0 = __builtin_amdgcn_rcpf(b);
+  const float y = __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
+  const float q0 = a * y;
+  return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
 }
 
 // Folds one nonzero (column j, log-weight lw, parameters r/beta/lc for the
 // lane's V samples) into the running argmins.  `ok` (wave-uniform) is false
-// for columns >= dim, which are never selected.
+// for columns >= dim, which are never selected.  `fast` (wave-uniform):
+// every r of the table is in [2^-60, 2^60] and lw is not +inf, so lw / r
+// may use div_rn_fast.
 template <int V>
 __device__ __forceinline__ void fold(const float (&r)[V], const float (&beta)[V],
                                      const float (&lc)[V], uint32_t j, float lw,
-                                     bool ok, Argmin<V> &acc) {
+                                     bool ok, bool fast, Argmin<V> &acc) {
   WMH_NO_CONTRACT
+  float q[V];
+  if (fast) {
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = div_rn_fast(lw, r[v]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = lw / r[v];
+  }
 #pragma unroll
   for (int v = 0; v < V; v++) {
-    const float t = __builtin_floorf(lw / r[v] + beta[v]);   // kernel.cu:103
+    const float t = __builtin_floorf(q[v] + beta[v]);        // kernel.cu:103
     const float ln_y = (t - beta[v]) * r[v];                 // kernel.cu:104
     const float ln_a = (lc[v] - ln_y) - r[v];                // kernel.cu:105
     const bool upd = ok && ln_a < acc.best[v];               // kernel.cu:106
@@ -218,8 +258,35 @@ struct LaneOffsets {
   uint32_t r, beta, lc;
 };
 
-__device__ __forceinline__ const float *at(const char *base, uint32_t off) {
-  return reinterpret_cast<const float *>(base + off);
+// A column block is read through a raw buffer resource held in SGPRs (the
+// block's address, its size as the range, dword 3 = 32-bit data format as
+// gfx9 requires), so each gather is one buffer load with the lane's byte
+// offset: no per-lane 64-bit address arithmetic.  Offsets past the block's
+// end read zero instead of memory.
+constexpr int kRawBufferWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const char *blk, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(blk), (short)0, (int)bytes,
+                                           kRawBufferWord3);
+}
+
+// Loads V consecutive floats at byte offset off of a block.
+template <int V>
+__device__ __forceinline__ void load_b(__amdgpu_buffer_rsrc_t rs, uint32_t off,
+                                       float (&out)[V]) {
+  if constexpr (V == 1) {
+    out[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+  } else if constexpr (V == 2) {
+    const float2 x = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+    out[0] = x.x;
+    out[1] = x.y;
+  } else {
+    const float4 x = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+    out[0] = x.x;
+    out[1] = x.y;
+    out[2] = x.z;
+    out[3] = x.w;
+  }
 }
 
 // Hashes CSR positions [begin, end) of one row into `acc` (CSR order, so a
@@ -231,7 +298,7 @@ __device__ __forceinline__ void hash_range(const float *__restrict__ params,
                                            LaneOffsets lo,
                                            const float *__restrict__ weights,
                                            const uint32_t *__restrict__ cols,
-                                           uint32_t begin, uint32_t end,
+                                           uint32_t begin, uint32_t end, bool fast_div,
                                            Argmin<V> &acc) {
   static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
   const uint32_t lane = lane_id();
@@ -244,6 +311,7 @@ __device__ __forceinline__ void hash_range(const float *__restrict__ params,
       c = cols[base + lane];
       lw = wmh_logf(weights[base + lane]);  // once per nonzero (kernel.cu:95)
     }
+    const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
     // The last group of a chunk may run past n: lanes >= n hold the column
     // WMH_SENTINEL (>= dim), so those slots read column 0's block and are
     // never selected, and the group keeps all of its gathers in flight.
@@ -255,13 +323,15 @@ __device__ __forceinline__ void hash_range(const float *__restrict__ params,
       for (int u = 0; u < U; u++) {
         j[u] = readlane_u32(c, i + u);
         lwu[u] = readlane_f32(lw, i + u);
-        const char *blk = pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes;
-        load_v<V>(at(blk, lo.r), r[u]);
-        load_v<V>(at(blk, lo.beta), beta[u]);
-        load_v<V>(at(blk, lo.lc), lc[u]);
+        const __amdgpu_buffer_rsrc_t rs =
+            block_rsrc(pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
+        load_b<V>(rs, lo.r, r[u]);
+        load_b<V>(rs, lo.beta, beta[u]);
+        load_b<V>(rs, lo.lc, lc[u]);
       }
 #pragma unroll
-      for (int u = 0; u < U; u++) fold<V>(r[u], beta[u], lc[u], j[u], lwu[u], j[u] < dim, acc);
+      for (int u = 0; u < U; u++)
+        fold<V>(r[u], beta[u], lc[u], j[u], lwu[u], j[u] < dim, fast, acc);
     }
   }
 }
@@ -307,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void wmh_plain_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples,
     uint32_t sp, uint32_t ntiles, const float *__restrict__ weights,
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ rows,
-    uint32_t length, uint32_t base0, uint32_t *__restrict__ output) {
+    uint32_t length, uint32_t base0, uint32_t *__restrict__ output, uint32_t fast_div) {
   const uint64_t total = (uint64_t)length * ntiles;
   const uint32_t lane = lane_id();
   // CSR positions are clamped to [rows[0], rows[length]], so a malformed
@@ -325,7 +395,8 @@ __global__ __launch_bounds__(kBlock) void wmh_plain_kernel(
     const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
     Argmin<V> acc;
     acc.init();
-    hash_range<V, WMH_GROUP_NNZ / V>(params, dim, 12ull * sp, lo, weights, cols, begin, end, acc);
+    hash_range<V, WMH_GROUP_NNZ / V>(params, dim, 12ull * sp, lo, weights, cols, begin, end,
+                                     fast_div != 0u, acc);
     if (k0 < samples) write_hashes<V>(output, row, samples, k0, acc);
   }
 }
@@ -423,7 +494,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
                                           LaneOffsets lo, const MatrixDesc &M,
                                           uint32_t a, uint32_t b, uint32_t row,
                                           uint32_t piece_start, uint32_t piece_end,
-                                          uint32_t skip_nnz) {
+                                          uint32_t skip_nnz, bool fast_div) {
   static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
   const uint32_t lane = lane_id();
   const char *pbase = reinterpret_cast<const char *>(params);
@@ -480,6 +551,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
       c = cols[cb + lane];
       lw = wmh_logf(weights[cb + lane]);
     }
+    const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
     // As in hash_range, the last group may run past n (slots >= n hold the
     // column WMH_SENTINEL and are never selected); only real positions can
     // end a row.  A striped row starting inside the chunk runs past its end
@@ -492,22 +564,24 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
       for (int u = 0; u < U; u++) {
         j[u] = readlane_u32(c, i + u);
         lwu[u] = readlane_f32(lw, i + u);
-        const char *blk = pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes;
-        load_v<V>(at(blk, lo.r), rr[u]);
-        load_v<V>(at(blk, lo.beta), beta[u]);
-        load_v<V>(at(blk, lo.lc), lc[u]);
+        const __amdgpu_buffer_rsrc_t rs =
+            block_rsrc(pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
+        load_b<V>(rs, lo.r, rr[u]);
+        load_b<V>(rs, lo.beta, beta[u]);
+        load_b<V>(rs, lo.lc, lc[u]);
       }
       const uint32_t p0 = cb + i;
       if (row_end > p0 + min((uint32_t)U, n - i)) {
         // the current row runs past the group: fold it without row checks
 #pragma unroll
         for (int u = 0; u < U; u++)
-          fold<V>(rr[u], beta[u], lc[u], split ? p0 + u : j[u], lwu[u], j[u] < dim, acc);
+          fold<V>(rr[u], beta[u], lc[u], split ? p0 + u : j[u], lwu[u], j[u] < dim, fast, acc);
       } else {
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const uint32_t p = p0 + u;
-          fold<V>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u], j[u] < dim && !skip, acc);
+          fold<V>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u], j[u] < dim && !skip, fast,
+                  acc);
           if (i + u < n && p + 1 == row_end) {
             flush_row<V>(M.output, r, samples, k0, split, acc);
             if (p + 1 < b) next_row();
@@ -569,7 +643,7 @@ template <int V>
 __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
-    StripePlan stripes) {
+    StripePlan stripes, uint32_t fast_div) {
   const uint32_t lane = lane_id();
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
@@ -655,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
         const uint32_t koff = (k0 < samples) ? k0 : 0u;
         const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
         hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row, ps, pe,
-                                         row == ~0u ? skip_nnz : 0u);
+                                         row == ~0u ? skip_nnz : 0u, fast_div != 0u);
       }
     }
   }
@@ -915,6 +989,17 @@ hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
   return hipGetLastError();
 }
 
+hipError_t launch_params_range(const float *params, uint32_t dim, uint32_t samples,
+                               uint32_t *bad, hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  const uint64_t total = (uint64_t)dim * samples;
+  uint64_t blocks = (total + kBlock - 1) / kBlock;
+  if (blocks > 65536u) blocks = 65536u;
+  hipLaunchKernelGGL(params_range_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
+                     dim3(kBlock), 0, stream, params, dim, samples, sp, bad);
+  return hipGetLastError();
+}
+
 hipError_t launch_scatter_params(const float *staging, uint32_t ncols,
                                  uint32_t j0, uint32_t which, float *params,
                                  uint32_t samples, hipStream_t stream) {
@@ -936,7 +1021,7 @@ hipError_t launch_gather_params(const float *params, uint32_t j0, uint32_t ncols
 }
 
 hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
-                        const MatrixDesc &m, int spl, hipStream_t stream) {
+                        const MatrixDesc &m, int spl, bool fast_div, hipStream_t stream) {
   const uint32_t sp = padded_samples(samples);
   const uint32_t per_wave = (uint32_t)(kWave * spl);
   const uint32_t ntiles = (samples + per_wave - 1) / per_wave;
@@ -945,17 +1030,17 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
     case 1:
       hipLaunchKernelGGL(wmh_plain_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, ntiles, m.weights, m.cols, m.rows,
-                         m.length, m.base, m.output);
+                         m.length, m.base, m.output, (uint32_t)fast_div);
       break;
     case 2:
       hipLaunchKernelGGL(wmh_plain_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, ntiles, m.weights, m.cols, m.rows,
-                         m.length, m.base, m.output);
+                         m.length, m.base, m.output, (uint32_t)fast_div);
       break;
     case 4:
       hipLaunchKernelGGL(wmh_plain_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, ntiles, m.weights, m.cols, m.rows,
-                         m.length, m.base, m.output);
+                         m.length, m.base, m.output, (uint32_t)fast_div);
       break;
     default:
       return hipErrorInvalidValue;
@@ -993,7 +1078,7 @@ hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t tot
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
-                             bool auto_seg, int num_cus, hipStream_t stream,
+                             bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
                              hipEvent_t ev_begin, hipEvent_t ev_end) {
   const uint32_t sp = padded_samples(samples);
   const int spl = tuning.spl;
@@ -1024,15 +1109,18 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   switch (spl) {
     case 1:
       hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, ctl, stripes);
+                         params, dim, samples, sp, mats, count, ctl, stripes,
+                         (uint32_t)fast_div);
       break;
     case 2:
       hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, ctl, stripes);
+                         params, dim, samples, sp, mats, count, ctl, stripes,
+                         (uint32_t)fast_div);
       break;
     default:
       hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, ctl, stripes);
+                         params, dim, samples, sp, mats, count, ctl, stripes,
+                         (uint32_t)fast_div);
       break;
   }
   if (ev_end) (void)hipEventRecord(ev_end, stream);

@@ -127,6 +127,7 @@ struct GraphCache {
   std::vector<MatrixDesc> key;
   int kernel = 0;
   Tuning tuning{0, 0, 0};
+  bool fast_div = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture = nullptr;
@@ -155,6 +156,10 @@ struct MinhashCudaGenerator_ {
   enum { kParamsNone = 0, kParamsSeeded = 1, kParamsTables = 2 };
   int params_source = kParamsNone;
   uint32_t seed = 0;
+  // every r of the table lies in [2^-60, 2^60], so the kernels may divide
+  // with the short correctly rounded sequence (refresh_division_mode)
+  bool fast_div = false;
+  wmh::DevBuf range_flag;
   hipStream_t stream = nullptr;
   std::mutex mu;
   // grow-only caches, reference minhashcuda.cc:27-31
@@ -196,5 +201,9 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
 // The hipGraph-replayed form of hash_device_batch for device-pointer calls.
 MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats, int kernel,
                         hipStream_t s);
+
+// Checks the r table on the device after the parameters change and sets
+// gen->fast_div (blocks until the check is done).
+MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen);
 
 }  // namespace wmh

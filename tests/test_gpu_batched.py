@@ -299,3 +299,52 @@ def test_random_batches_and_tunings(libmhcuda, capi, seed):
     for i, (h, m) in enumerate(zip(got, mats)):
         ref = oracle.calc(*params, m.data, m.indices.astype(np.uint32), m.indptr)
         assert np.array_equal(h, ref), (seed, i)
+
+
+@pytest.mark.parametrize("case", ["near_range_ends", "below_range", "above_range", "nan_r"])
+@pytest.mark.parametrize("kernel", [PLAIN, PERSISTENT])
+def test_division_modes(libmhcuda, capi, case, kernel):
+    """ln(w) / r is computed with a short correctly rounded sequence when every
+    r of the table lies in [2^-60, 2^60] and with the general IEEE sequence
+    otherwise; both must give the oracle's hashes.  Tables with every r near
+    one end of that range (short sequence), and tables with one r just below
+    it, just above it or NaN (general sequence).  Weights include 1 (ln w =
+    0), 0, subnormals, the largest floats, NaN, negatives, and +inf, whose
+    chunk of nonzeros takes the IEEE sequence in either mode."""
+    say("case", case, "kernel", kernel)
+    dim, samples = 3000, 64
+    rng = np.random.default_rng(41)
+    rs = rng.gamma(2.0, 1.0, (samples, dim)).astype(np.float32)
+    if case == "near_range_ends":
+        e = rng.choice([-60, -59, -58, 57, 58, 59], (samples, dim))
+        rs = np.ldexp(rng.uniform(1.0, 2.0, (samples, dim)), e).astype(np.float32)
+        assert rs.min() >= 2.0 ** -60 and rs.max() <= 2.0 ** 60
+    elif case == "below_range":
+        rs[5, 17] = np.float32(2.0 ** -61)
+    elif case == "above_range":
+        rs[5, 17] = np.float32(2.0 ** 61)
+    else:
+        rs[5, 17] = np.nan
+    ln_cs = np.log(rng.gamma(2.0, 1.0, (samples, dim))).astype(np.float32)
+    betas = rng.uniform(0, 1, (samples, dim)).astype(np.float32)
+    nnz = rng.integers(1, 3000, 40)
+    m = workloads.batched_csr(nnz.astype(np.int64), dim=dim, sigma=3.0, seed=7)
+    w = m.data
+    special = np.array([1.0, 0.0, 1e-42, 3e38, np.nan, -2.0, 1.0, np.float32(1 - 2 ** -24)],
+                       np.float32)
+    pick = rng.choice(m.nnz, 400, replace=False)
+    w[pick] = special[rng.integers(0, len(special), 400)]
+    w[m.indptr[3] + 5] = np.inf   # row 3: a +inf weight (selected in every sample)
+    w[m.indptr[30]:m.indptr[31]] = 1.0  # row 30: every ln w = 0
+    ref = oracle.calc(rs, ln_cs, betas, m.data, m.indices, m.indptr)
+    gen = libmhcuda.minhash_cuda_init(dim, samples, deferred=True)
+    libmhcuda.minhash_cuda_assign_vars(gen, rs, ln_cs, betas)
+    g = ctypes.c_void_p(gen)
+    assert capi.mhcuda_set_stripes(g, 1) == 0
+    try:
+        got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=kernel)[0]
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+    c_inf = m.indices[m.indptr[3] + 5]
+    assert (ref[3, np.isfinite(rs[:, c_inf]), 0] == c_inf).all()
+    assert np.array_equal(got, ref)
