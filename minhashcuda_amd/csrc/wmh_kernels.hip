@@ -23,6 +23,7 @@
 
 #include <algorithm>
 
+#include "wmh_div.h"
 #include "wmh_kernels.h"
 #include "wmh_math.h"
 
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void params_range_kernel(const float *__res
       const uint32_t j = (uint32_t)(idx / samples);
       const uint32_t k = (uint32_t)(idx - (uint64_t)j * samples);
       const float r = params[(uint64_t)j * 3u * sp + k];
-      out = !(r >= 0x1p-60f && r <= 0x1p60f);
+      out = !(r >= kFastDivMinB && r <= kFastDivMaxB);
     }
     if (__ballot(out) != 0ull && lane_id() == 0) atomicOr(bad, 1u);
   }
@@ -199,29 +200,6 @@ struct Argmin {
 #ifndef WMH_GROUP_NNZ
 #define WMH_GROUP_NNZ 16
 #endif
-
-// a / b rounded to nearest, in 6 instructions instead of the 11 of the
-// general IEEE sequence (which also scales operands near the ends of the
-// exponent range and fixes up zeros, infinities and NaNs).  Valid for every
-// b in [2^-60, 2^60] and every finite a with |a| in [2^-25, 2^7] or a = +0,
-// which covers a = ln(w) for every positive finite float w:
-//   y = one Newton step from v_rcp_f32 is RN(1/b) for every such b (checked
-//       exhaustively on gfx950 over all 1,015,021,568 floats in
-//       [2^-60, 2^61));
-//   q0 = RN(a * y) is within 1 ulp of a / b, so rem = a - b * q0 is exact
-//       (no underflow in this range) and RN(q0 + rem * y) = RN(a / b)
-//       (P. Markstein, IBM J. Res. Dev. 34(1), 1990; J.-M. Muller et al.,
-//       Handbook of Floating-Point Arithmetic, on FMA-based division);
-//       also checked exhaustively on gfx950 against IEEE division for all
-//       2^46 significand pairs at two exponent pairs.
-// For a = -inf or NaN both give a value (-inf / NaN vs NaN) that is never
-// selected; a = +inf must take the IEEE path (+inf vs NaN).
-__device__ __forceinline__ float div_rn_fast(float a, float b) {
-  const float y0 = __builtin_amdgcn_rcpf(b);
-  const float y = __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
-  const float q0 = a * y;
-  return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
-}
 
 // Folds one nonzero (column j, log-weight lw, parameters r/beta/lc for the
 // lane's V samples) into the running argmins.  `ok` (wave-uniform) is false
