@@ -32,9 +32,9 @@ __device__ __forceinline__ float recip_rn(float b) {
 // Also checked exhaustively on gfx950 against IEEE division for all 2^46
 // significand pairs at the exponent pairs (a, b) in ([1, 2), [1, 2)),
 // ([2^-20, 2^-19), [2^30, 2^31)), ([64, 128), [2^-60, 2^-59)) and
-// ([2^-25, 2^-24), [2^59, 2^60)).  For a = -inf or NaN the IEEE quotient (-inf or NaN) and this
-// one (NaN) both give a hash candidate that is never selected; a = +inf must
-// take the IEEE path (+inf here is NaN).
+// ([2^-25, 2^-24), [2^59, 2^60)).  For a = -inf or NaN the IEEE quotient
+// (-inf or NaN) and this one (NaN) both give a hash candidate that is never
+// selected; a = +inf must take the IEEE path (+inf here is NaN).
 __device__ __forceinline__ float div_rn_fast(float a, float b) {
   const float y = recip_rn(b);
   const float q0 = a * y;
