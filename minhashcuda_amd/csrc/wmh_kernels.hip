@@ -572,6 +572,159 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   if (row_end > b && !skip) flush_row<V>(M.output, r, samples, k0, true, acc);  // continues later
 }
 
+// Stripe pieces of a 128-sample tile, two nonzeros per step: lanes 0-31
+// take the even offsets of [a, b) and lanes 32-63 the odd ones, each lane 4
+// samples (k0 = 128 tile + 4 (lane % 32)), so every gather is one full-wave
+// 16-byte-per-lane load (3 per step of two nonzeros instead of 3 per
+// nonzero).  Within a half the positions are folded in CSR order (strict <);
+// at the end the halves merge by (ln_a, position), which keeps the earliest
+// of equal minima, and lanes 0-31 merge the result into the row's keyed
+// minima like any piece of a split row.  A chunk's column offsets and
+// log-weights reach the lanes of each half through ds_bpermute; a column >=
+// dim gets the log-weight NaN, which is never selected.  Gathers go through
+// a buffer resource at the stripe's first column block with 32-bit offsets;
+// a chunk holding a column too far from the stripe for that (an unsorted row
+// in a table of 4 GiB or more) uses 64-bit addresses instead.
+constexpr int kPairSteps = 4;  // pairs of nonzeros whose gathers are issued together
+
+__device__ __forceinline__ uint32_t bperm_u32(uint32_t v, uint32_t src_lane) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
+}
+
+// 16 bytes at voffset + soffset (soffset wave-uniform, in an SGPR)
+__device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
+}
+
+__device__ __forceinline__ void fold4(const float4 &r4, const float4 &b4, const float4 &c4,
+                                      float lw, uint32_t p, bool fast, float (&best)[4],
+                                      uint32_t (&pos)[4]) {
+  WMH_NO_CONTRACT
+  const float r[4] = {r4.x, r4.y, r4.z, r4.w};
+  const float beta[4] = {b4.x, b4.y, b4.z, b4.w};
+  const float lc[4] = {c4.x, c4.y, c4.z, c4.w};
+  float q[4];
+  if (fast) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) q[v] = div_rn_fast(lw, r[v]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; v++) q[v] = lw / r[v];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const float t = __builtin_floorf(q[v] + beta[v]);     // kernel.cu:103
+    const float ln_y = (t - beta[v]) * r[v];              // kernel.cu:104
+    const float ln_a = (lc[v] - ln_y) - r[v];             // kernel.cu:105
+    const bool upd = ln_a < best[v];                      // kernel.cu:106
+    best[v] = upd ? ln_a : best[v];
+    pos[v] = upd ? p : pos[v];
+  }
+}
+
+__device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ params, uint32_t dim,
+                                                 uint32_t sp, uint32_t samples, uint32_t tile,
+                                                 const MatrixDesc &M, uint32_t a, uint32_t b,
+                                                 uint32_t row, uint32_t col0_in, bool fast_div) {
+  const uint32_t lane = lane_id();
+  const uint32_t half = lane >> 5;
+  const uint32_t col0 = uniform(col0_in);
+  const uint32_t k0 = tile * 128u + 4u * (lane & 31u);
+  const uint32_t koff = k0 < samples ? k0 : 0u;
+  const uint32_t block_bytes = 12u * sp;
+  const uint32_t off_r = koff * 4u;              // lane part of the offset
+  const uint32_t s_b = uniform(sp * 4u), s_c = uniform(sp * 8u);  // beta, ln_c arrays
+  // column offsets c - col0 up to `span` keep every lane's 16 bytes below 2^32
+  const uint32_t span = (0xffffffffu - 12u * sp) / block_bytes;
+  // the resource must live in SGPRs: its base is made wave-uniform explicitly
+  const uint64_t sb = reinterpret_cast<uint64_t>(params) + (uint64_t)col0 * block_bytes;
+  char *sbase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(sb >> 32)) << 32) |
+                                         uniform((uint32_t)sb));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(sbase, (short)0, (int)0xffffffff, kRawBufferWord3);
+  const uint32_t *__restrict__ cols = M.cols - M.base;
+  const float *__restrict__ weights = M.weights - M.base;
+  float best[4];
+  uint32_t pos[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    best[v] = FLT_MAX;
+    pos[v] = WMH_SENTINEL;
+  }
+  for (uint32_t cb = a; cb < b; cb += kWave) {
+    const uint32_t n = uniform(min((uint32_t)kWave, b - cb));
+    // lanes without a selectable nonzero read a valid block (the stripe's
+    // first, or column 0 for a stripe past dim) and are never selected
+    uint32_t c = col0 < dim ? col0 : 0u;
+    float lw = __builtin_nanf("");
+    if (lane < n) {
+      const uint32_t cc = cols[cb + lane];
+      if (cc < dim) {
+        c = cc;
+        lw = wmh_logf(weights[cb + lane]);  // once per nonzero (kernel.cu:95)
+      }
+    }
+    const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
+    const uint32_t rel = c - col0;
+    const bool fits = __ballot(rel > span) == 0ull;
+    const uint32_t o = rel * block_bytes;
+    // 2 kPairSteps divides 64, so a group never reaches past lane 63; lanes
+    // past n hold NaN log-weights
+    for (uint32_t i = 0; i < n; i += 2 * kPairSteps) {
+      float4 r4[kPairSteps], b4[kPairSteps], c4[kPairSteps];
+      float lws[kPairSteps];
+      if (fits) {
+#pragma unroll
+        for (int u = 0; u < kPairSteps; u++) {
+          const uint32_t src = i + 2u * u + half;
+          const uint32_t ou = bperm_u32(o, src) + off_r;
+          lws[u] = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
+          r4[u] = load4(rs, ou, 0u);
+          b4[u] = load4(rs, ou, s_b);
+          c4[u] = load4(rs, ou, s_c);
+        }
+      } else {
+        const char *pbase = reinterpret_cast<const char *>(params);
+#pragma unroll
+        for (int u = 0; u < kPairSteps; u++) {
+          const uint32_t src = i + 2u * u + half;
+          const uint32_t cu = bperm_u32(c, src);
+          lws[u] = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
+          const char *blk = pbase + (uint64_t)cu * block_bytes + off_r;
+          r4[u] = *reinterpret_cast<const float4 *>(blk);
+          b4[u] = *reinterpret_cast<const float4 *>(blk + s_b);
+          c4[u] = *reinterpret_cast<const float4 *>(blk + s_c);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kPairSteps; u++)
+        fold4(r4[u], b4[u], c4[u], lws[u], cb + i + 2u * u + half, fast, best, pos);
+    }
+  }
+  // merge the odd positions (lanes 32-63) into the even ones (lanes 0-31)
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const float ob = __uint_as_float(bperm_u32(__float_as_uint(best[v]), lane ^ 32u));
+    const uint32_t op = bperm_u32(pos[v], lane ^ 32u);
+    const bool take = ob < best[v] || (ob == best[v] && op < pos[v]);
+    best[v] = take ? ob : best[v];
+    pos[v] = take ? op : pos[v];
+  }
+  if (half != 0u || k0 >= samples) return;
+  unsigned long long *slot =
+      reinterpret_cast<unsigned long long *>(M.output) + ((uint64_t)row * samples + k0);
+  unsigned long long seen[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) seen[v] = k0 + v < samples ? slot[v] : 0ull;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    if (k0 + v < samples && pos[v] != WMH_SENTINEL) {
+      const unsigned long long key = ((unsigned long long)order_key(best[v]) << 32) | pos[v];
+      if (key < seen[v]) atomicMin(slot + v, key);
+    }
+  }
+}
+
 // Stripe piece k (of K) of the CSR positions [s, e): from the first position
 // whose column is >= k * width to the first whose column is >= (k + 1) *
 // width, or to e for the last stripe.  The two binary searches run together.
@@ -703,6 +856,14 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
           row = ~0u;
         }
         const MatrixDesc M = mats[m];
+        if constexpr (V == 2) {
+          if (piece) {
+            const uint64_t col0 = (uint64_t)k * stripes.width;
+            hash_piece_pairs(params, dim, sp, samples, tile, M, a, b, row,
+                             col0 < dim ? (uint32_t)col0 : dim, fast_div != 0u);
+            continue;
+          }
+        }
         const uint32_t k0 = tile * (kWave * V) + lane * V;
         const uint32_t koff = (k0 < samples) ? k0 : 0u;
         const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
