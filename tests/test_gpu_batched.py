@@ -348,3 +348,62 @@ def test_division_modes(libmhcuda, capi, case, kernel):
     c_inf = m.indices[m.indptr[3] + 5]
     assert (ref[3, np.isfinite(rs[:, c_inf]), 0] == c_inf).all()
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("samples", [2, 100, 200, 256])
+def test_paired_piece_shapes(libmhcuda, capi, samples):
+    """Stripe pieces at 2 samples per lane are hashed two nonzeros per step
+    (a half-wave per nonzero, 4 samples per lane, 128-sample tiles): sample
+    counts that leave a tile partly empty or use several tiles, sorted and
+    unsorted striped rows of odd and even lengths, columns >= dim, and short
+    rows between them."""
+    say("samples", samples)
+    dim = max(20000, (3 << 20) // (12 * ((samples + 3) // 4 * 4)))
+    rng = np.random.default_rng(samples)
+    nnz = np.concatenate([rng.integers(1100, 5000, 12), rng.integers(1, 300, 30)])
+    rng.shuffle(nnz)
+    m = workloads.batched_csr(nnz.astype(np.int64), dim=dim + 7, sigma=1.5, seed=samples)
+    m = shuffled_rows(m, 1) if samples in (100, 256) else m
+    m.data[rng.random(m.nnz) < 0.02] = 0.0
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=samples + 1)
+    params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+    g = ctypes.c_void_p(gen)
+    assert capi.mhcuda_set_stripes(g, 1) == 0
+    k = ctypes.c_uint32()
+    assert capi.mhcuda_get_stripes(g, None, ctypes.byref(k), None) == 0
+    assert k.value > 1
+    ref = oracle.calc(*params, m.data, m.indices.astype(np.uint32), m.indptr)
+    try:
+        for seg in (256, 1024):
+            assert capi.mhcuda_set_tuning(g, 2, seg, 0) == 0
+            got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+            assert np.array_equal(got, ref), seg
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+
+
+def test_paired_pieces_wide_table(libmhcuda, capi):
+    """A 4.6 GB parameter table (3M columns x 128 samples): the stripe pieces
+    of an unsorted row hold columns more than 2^32 bytes of parameters away
+    from their stripe, so their chunks gather with 64-bit addresses, while the
+    sorted rows gather through 32-bit offsets from the stripe's start."""
+    dim, samples = 3_000_000, 128
+    rng = np.random.default_rng(5)
+    nnz = np.array([5000, 1200, 40, 4097, 3001], np.int64)
+    m = workloads.batched_csr(nnz, dim=dim, sigma=1.0, seed=5)
+    s, e = m.indptr[0], m.indptr[1]
+    assert e - s == 5000
+    perm = rng.permutation(e - s)
+    m.indices[s:e] = m.indices[s:e][perm]       # row 0 unsorted: far columns in every piece
+    m.data[s:e] = m.data[s:e][perm]
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=9)
+    g = ctypes.c_void_p(gen)
+    try:
+        assert capi.mhcuda_set_stripes(g, 4) == 0
+        assert capi.mhcuda_set_tuning(g, 2, 1024, 0) == 0
+        got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+    ref = oracle.calc(*params, m.data, m.indices.astype(np.uint32), m.indptr)
+    assert np.array_equal(got, ref)
