@@ -256,6 +256,13 @@ def serve_bench(args, torch):
             arrs.append((ot, ot.numpy().view(np.uint32)))
             pinned_pool.append(arrs)
     gen = libMHCUDA.minhash_cuda_init(dim, samples, seed=1, devices=1)
+    if explicit_tuning(args):
+        from minhashcuda_amd import capi as capi_mod
+        capi = capi_mod.load(os.path.join(PKG, "libMHCUDA.so"))
+        g = ctypes.c_void_p(gen)
+        capi_mod.check(capi.mhcuda_set_tuning(g, args.spl, args.seg, args.wpc), "tuning")
+        if args.stripe_mib >= 0:
+            capi_mod.check(capi.mhcuda_set_stripes(g, args.stripe_mib), "stripes")
     server = serving.Server(gen, memory_budget=4 << 30, slots=args.slots, max_batch_requests=256)
     # warm-up: every buffer reaches its working size
     for t in [server.submit(m) for m in pool[:64]]:

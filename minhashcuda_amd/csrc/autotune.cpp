@@ -188,7 +188,8 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
         consider(c);
       }
       const Candidate after_seg = best;
-      for (int wpc : {8, 16, 24, 32}) {
+      // (the hashing kernels hold at most 4 waves per SIMD, 16 per CU)
+      for (int wpc : {8, 12, 16}) {
         if (wpc == after_seg.tuning.waves_per_cu) continue;
         Candidate c = after_seg;
         c.tuning.waves_per_cu = wpc;

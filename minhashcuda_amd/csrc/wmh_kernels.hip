@@ -22,6 +22,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "wmh_div.h"
 #include "wmh_kernels.h"
@@ -201,27 +202,36 @@ struct Argmin {
 #define WMH_GROUP_NNZ 16
 #endif
 
+// lw / r: the short sequence (Fast: every r of the table is in
+// [kFastDivMinB, kFastDivMaxB] and no lw of the chunk is +inf) or IEEE.
+template <bool Fast>
+__device__ __forceinline__ float div_lw(float lw, float r) {
+  if constexpr (Fast) return div_rn_fast(lw, r);
+  else return lw / r;
+}
+
+// Runs body(std::true_type) when `fast` (wave-uniform) holds, else
+// body(std::false_type): the division mode is chosen once per chunk of
+// nonzeros, outside the loops that keep gathers in flight (a branch per fold
+// would join the two division sequences before every fold and make the
+// compiler wait there for every outstanding load and store).
+template <typename Body>
+__device__ __forceinline__ void with_division(bool fast, Body &&body) {
+  if (fast) body(std::true_type{});
+  else body(std::false_type{});
+}
+
 // Folds one nonzero (column j, log-weight lw, parameters r/beta/lc for the
 // lane's V samples) into the running argmins.  `ok` (wave-uniform) is false
-// for columns >= dim, which are never selected.  `fast` (wave-uniform):
-// every r of the table is in [2^-60, 2^60] and lw is not +inf, so lw / r
-// may use div_rn_fast.
-template <int V>
+// for columns >= dim, which are never selected.
+template <int V, bool Fast>
 __device__ __forceinline__ void fold(const float (&r)[V], const float (&beta)[V],
                                      const float (&lc)[V], uint32_t j, float lw,
-                                     bool ok, bool fast, Argmin<V> &acc) {
+                                     bool ok, Argmin<V> &acc) {
   WMH_NO_CONTRACT
-  float q[V];
-  if (fast) {
-#pragma unroll
-    for (int v = 0; v < V; v++) q[v] = div_rn_fast(lw, r[v]);
-  } else {
-#pragma unroll
-    for (int v = 0; v < V; v++) q[v] = lw / r[v];
-  }
 #pragma unroll
   for (int v = 0; v < V; v++) {
-    const float t = __builtin_floorf(q[v] + beta[v]);        // kernel.cu:103
+    const float t = __builtin_floorf(div_lw<Fast>(lw, r[v]) + beta[v]);  // kernel.cu:103
     const float ln_y = (t - beta[v]) * r[v];                 // kernel.cu:104
     const float ln_a = (lc[v] - ln_y) - r[v];                // kernel.cu:105
     const bool upd = ok && ln_a < acc.best[v];               // kernel.cu:106
@@ -293,24 +303,26 @@ __device__ __forceinline__ void hash_range(const float *__restrict__ params,
     // The last group of a chunk may run past n: lanes >= n hold the column
     // WMH_SENTINEL (>= dim), so those slots read column 0's block and are
     // never selected, and the group keeps all of its gathers in flight.
-    for (uint32_t i = 0; i < n; i += U) {
-      uint32_t j[U];
-      float lwu[U];
-      float r[U][V], beta[U][V], lc[U][V];
+    with_division(fast, [&](auto mode) {
+      for (uint32_t i = 0; i < n; i += U) {
+        uint32_t j[U];
+        float lwu[U];
+        float r[U][V], beta[U][V], lc[U][V];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        j[u] = readlane_u32(c, i + u);
-        lwu[u] = readlane_f32(lw, i + u);
-        const __amdgpu_buffer_rsrc_t rs =
-            block_rsrc(pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
-        load_b<V>(rs, lo.r, r[u]);
-        load_b<V>(rs, lo.beta, beta[u]);
-        load_b<V>(rs, lo.lc, lc[u]);
+        for (int u = 0; u < U; u++) {
+          j[u] = readlane_u32(c, i + u);
+          lwu[u] = readlane_f32(lw, i + u);
+          const __amdgpu_buffer_rsrc_t rs = block_rsrc(
+              pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
+          load_b<V>(rs, lo.r, r[u]);
+          load_b<V>(rs, lo.beta, beta[u]);
+          load_b<V>(rs, lo.lc, lc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          fold<V, decltype(mode)::value>(r[u], beta[u], lc[u], j[u], lwu[u], j[u] < dim, acc);
       }
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        fold<V>(r[u], beta[u], lc[u], j[u], lwu[u], j[u] < dim, fast, acc);
-    }
+    });
   }
 }
 
@@ -534,39 +546,43 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
     // column WMH_SENTINEL and are never selected); only real positions can
     // end a row.  A striped row starting inside the chunk runs past its end
     // (it has at least 64 nonzeros): its slots are not folded.
-    for (uint32_t i = 0; i < n && !skip; i += U) {
-      uint32_t j[U];
-      float lwu[U];
-      float rr[U][V], beta[U][V], lc[U][V];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        j[u] = readlane_u32(c, i + u);
-        lwu[u] = readlane_f32(lw, i + u);
-        const __amdgpu_buffer_rsrc_t rs =
-            block_rsrc(pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
-        load_b<V>(rs, lo.r, rr[u]);
-        load_b<V>(rs, lo.beta, beta[u]);
-        load_b<V>(rs, lo.lc, lc[u]);
-      }
-      const uint32_t p0 = cb + i;
-      if (row_end > p0 + min((uint32_t)U, n - i)) {
-        // the current row runs past the group: fold it without row checks
-#pragma unroll
-        for (int u = 0; u < U; u++)
-          fold<V>(rr[u], beta[u], lc[u], split ? p0 + u : j[u], lwu[u], j[u] < dim, fast, acc);
-      } else {
+    with_division(fast, [&](auto mode) {
+      constexpr bool kFast = decltype(mode)::value;
+      for (uint32_t i = 0; i < n && !skip; i += U) {
+        uint32_t j[U];
+        float lwu[U];
+        float rr[U][V], beta[U][V], lc[U][V];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-          const uint32_t p = p0 + u;
-          fold<V>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u], j[u] < dim && !skip, fast,
-                  acc);
-          if (i + u < n && p + 1 == row_end) {
-            flush_row<V>(M.output, r, samples, k0, split, acc);
-            if (p + 1 < b) next_row();
+          j[u] = readlane_u32(c, i + u);
+          lwu[u] = readlane_f32(lw, i + u);
+          const __amdgpu_buffer_rsrc_t rs = block_rsrc(
+              pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
+          load_b<V>(rs, lo.r, rr[u]);
+          load_b<V>(rs, lo.beta, beta[u]);
+          load_b<V>(rs, lo.lc, lc[u]);
+        }
+        const uint32_t p0 = cb + i;
+        if (row_end > p0 + min((uint32_t)U, n - i)) {
+          // the current row runs past the group: fold it without row checks
+#pragma unroll
+          for (int u = 0; u < U; u++)
+            fold<V, kFast>(rr[u], beta[u], lc[u], split ? p0 + u : j[u], lwu[u], j[u] < dim,
+                           acc);
+        } else {
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const uint32_t p = p0 + u;
+            fold<V, kFast>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u],
+                           j[u] < dim && !skip, acc);
+            if (i + u < n && p + 1 == row_end) {
+              flush_row<V>(M.output, r, samples, k0, split, acc);
+              if (p + 1 < b) next_row();
+            }
           }
         }
       }
-    }
+    });
     cb += n;
   }
   if (row_end > b && !skip) flush_row<V>(M.output, r, samples, k0, true, acc);  // continues later
@@ -596,24 +612,17 @@ __device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t rs, uint32_t voff
   return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
 }
 
+template <bool Fast>
 __device__ __forceinline__ void fold4(const float4 &r4, const float4 &b4, const float4 &c4,
-                                      float lw, uint32_t p, bool fast, float (&best)[4],
+                                      float lw, uint32_t p, float (&best)[4],
                                       uint32_t (&pos)[4]) {
   WMH_NO_CONTRACT
   const float r[4] = {r4.x, r4.y, r4.z, r4.w};
   const float beta[4] = {b4.x, b4.y, b4.z, b4.w};
   const float lc[4] = {c4.x, c4.y, c4.z, c4.w};
-  float q[4];
-  if (fast) {
-#pragma unroll
-    for (int v = 0; v < 4; v++) q[v] = div_rn_fast(lw, r[v]);
-  } else {
-#pragma unroll
-    for (int v = 0; v < 4; v++) q[v] = lw / r[v];
-  }
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const float t = __builtin_floorf(q[v] + beta[v]);     // kernel.cu:103
+    const float t = __builtin_floorf(div_lw<Fast>(lw, r[v]) + beta[v]);  // kernel.cu:103
     const float ln_y = (t - beta[v]) * r[v];              // kernel.cu:104
     const float ln_a = (lc[v] - ln_y) - r[v];             // kernel.cu:105
     const bool upd = ln_a < best[v];                      // kernel.cu:106
@@ -670,36 +679,39 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
     const uint32_t o = rel * block_bytes;
     // 2 kPairSteps divides 64, so a group never reaches past lane 63; lanes
     // past n hold NaN log-weights
-    for (uint32_t i = 0; i < n; i += 2 * kPairSteps) {
-      float4 r4[kPairSteps], b4[kPairSteps], c4[kPairSteps];
-      float lws[kPairSteps];
-      if (fits) {
+    // one loop per (division mode, addressing) so that no branch sits
+    // between a group's gathers and its folds
+    auto pairs = [&](auto mode, auto near) {
+      for (uint32_t i = 0; i < n; i += 2 * kPairSteps) {
+        float4 r4[kPairSteps], b4[kPairSteps], c4[kPairSteps];
+        float lws[kPairSteps];
 #pragma unroll
         for (int u = 0; u < kPairSteps; u++) {
           const uint32_t src = i + 2u * u + half;
-          const uint32_t ou = bperm_u32(o, src) + off_r;
           lws[u] = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
-          r4[u] = load4(rs, ou, 0u);
-          b4[u] = load4(rs, ou, s_b);
-          c4[u] = load4(rs, ou, s_c);
+          if constexpr (decltype(near)::value) {
+            const uint32_t ou = bperm_u32(o, src) + off_r;
+            r4[u] = load4(rs, ou, 0u);
+            b4[u] = load4(rs, ou, s_b);
+            c4[u] = load4(rs, ou, s_c);
+          } else {
+            const char *blk = reinterpret_cast<const char *>(params) +
+                              (uint64_t)bperm_u32(c, src) * block_bytes + off_r;
+            r4[u] = *reinterpret_cast<const float4 *>(blk);
+            b4[u] = *reinterpret_cast<const float4 *>(blk + s_b);
+            c4[u] = *reinterpret_cast<const float4 *>(blk + s_c);
+          }
         }
-      } else {
-        const char *pbase = reinterpret_cast<const char *>(params);
 #pragma unroll
-        for (int u = 0; u < kPairSteps; u++) {
-          const uint32_t src = i + 2u * u + half;
-          const uint32_t cu = bperm_u32(c, src);
-          lws[u] = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
-          const char *blk = pbase + (uint64_t)cu * block_bytes + off_r;
-          r4[u] = *reinterpret_cast<const float4 *>(blk);
-          b4[u] = *reinterpret_cast<const float4 *>(blk + s_b);
-          c4[u] = *reinterpret_cast<const float4 *>(blk + s_c);
-        }
+        for (int u = 0; u < kPairSteps; u++)
+          fold4<decltype(mode)::value>(r4[u], b4[u], c4[u], lws[u], cb + i + 2u * u + half,
+                                       best, pos);
       }
-#pragma unroll
-      for (int u = 0; u < kPairSteps; u++)
-        fold4(r4[u], b4[u], c4[u], lws[u], cb + i + 2u * u + half, fast, best, pos);
-    }
+    };
+    with_division(fast, [&](auto mode) {
+      if (fits) pairs(mode, std::true_type{});
+      else pairs(mode, std::false_type{});
+    });
   }
   // merge the odd positions (lanes 32-63) into the even ones (lanes 0-31)
 #pragma unroll
@@ -727,12 +739,14 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
 
 // Stripe piece k (of K) of the CSR positions [s, e): from the first position
 // whose column is >= k * width to the first whose column is >= (k + 1) *
-// width, or to e for the last stripe.  The two binary searches run together.
-// Whatever the column order, a search's result never decreases as its key
-// grows (two keys take the same path until the first probed column lying
-// between them, where the smaller key turns left and the larger right), so
-// the end of piece k is the start of piece k + 1 and the K pieces partition
-// [s, e).
+// width, or to e for the last stripe.  The two binary searches run together
+// (a wider search needs fewer rounds but more loads, and the gathers that
+// bound the kernel compete with them for the texture addresser; units of
+// several pieces hide the rounds' latency instead).  Whatever the column
+// order, a search's result never decreases as its key grows (two keys take
+// the same path until the first probed column lying between them, where the
+// smaller key turns left and the larger right), so the end of piece k is the
+// start of piece k + 1 and the K pieces partition [s, e).
 __device__ __forceinline__ void stripe_piece(const uint32_t *__restrict__ cols, uint32_t s,
                                              uint32_t e, uint32_t k, uint32_t K,
                                              uint32_t width, uint32_t &a, uint32_t &b) {
@@ -792,11 +806,14 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   const uint64_t pieces = (uint64_t)uniform(ctl->n_striped) * ntiles;  // per stripe
   const uint32_t K = stripes.stripes > 1 ? stripes.stripes : 0u;
   const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
-  // at most stripes.per_unit, and few enough that every stripe has several
-  // units per wavefront of its XCD
   const uint32_t Q = stripes.queues;
+  // pieces per unit: at most stripes.per_unit, and few enough that every
+  // stripe has several units per wavefront of its XCD, but at least 8 when
+  // stripes.per_unit allows (each unit starts with the searches for its
+  // pieces' bounds, whose latency fewer pieces do not hide)
   const uint32_t per_unit = (uint32_t)max(
-      1ull, min((unsigned long long)stripes.per_unit, pieces * Q / (4ull * waves)));
+      (unsigned long long)min(8u, stripes.per_unit),
+      min((unsigned long long)stripes.per_unit, pieces * Q / (4ull * waves)));
   const uint64_t per_stripe = (pieces + per_unit - 1) / per_unit;
   const uint64_t item_units = uniform(ctl->total_units);
   const uint32_t skip_nnz = K ? stripes.min_nnz : 0u;

@@ -115,7 +115,7 @@ struct PlanCtx {
 // A tuning chosen by the autotuner for one batch shape.
 struct TunedChoice {
   int kernel = 0;
-  Tuning tuning{2, 2048, 16, kDefaultStripeMib};
+  Tuning tuning{2, 2048, 12, kDefaultStripeMib};
   float ms = 0;           // time of the timed prefix
   uint64_t sample_nnz = 0;
 };
@@ -171,7 +171,10 @@ struct MinhashCudaGenerator_ {
   hipStream_t copy_stream = nullptr, d2h_stream = nullptr;
   uint64_t piece_nnz = 1ull << 28;
   uint64_t piece_rows = 1ull << 20;
-  wmh::Tuning tuning{2, 2048, 16, wmh::kDefaultStripeMib};
+  // 12 waves per CU by default: 3 per SIMD leave room in the register file
+  // for the runtime's copy kernels of overlapped uploads and downloads (the
+  // hashing kernels use up to 128 VGPRs, so 4 waves per SIMD fill it)
+  wmh::Tuning tuning{2, 2048, 12, wmh::kDefaultStripeMib};
   bool seg_auto = true;  // let small batches use smaller work items
   wmh::GraphCache graph;  // hipGraph replay of device-pointer calls (opt-in)
   std::map<uint64_t, wmh::TunedChoice> tuned;  // autotuner cache by batch shape
