@@ -162,12 +162,12 @@ class Engine:
         self.L.minhash_cuda_fini(self.gen)
 
 
-def measured_traffic(workload, kernel, alg_bytes, stripe_mib=None):
+def measured_traffic(workload, kernel, alg_bytes, stripes=None):
     """HBM bytes per launch of `kernel` on this workload as measured by the
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
     profiles/hbm_traffic.json (scaled by a copy-kernel calibration); None when
     no measurement of this exact configuration (input size, kernel and, for
-    the persistent kernel, column-stripe size) is recorded."""
+    the persistent kernel, number of column stripes) is recorded."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
@@ -177,7 +177,7 @@ def measured_traffic(workload, kernel, alg_bytes, stripe_mib=None):
     for rec in recs:
         if rec.get("kernel") != kernel or rec.get("algorithmic_bytes") != alg_bytes:
             continue
-        if stripe_mib is not None and rec.get("stripe_mib") != stripe_mib:
+        if stripes is not None and rec.get("stripes") != stripes:
             continue
         return rec["traffic_bytes"]
     return None
@@ -470,7 +470,7 @@ def large_bench(args, torch):
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("large", kname, alg,
-                                                 engine.tuning()["stripe_mib"]
+                                                 engine.tuning()["stripes"]
                                                  if kname == "wmh_persistent_kernel" else None),
                      "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms},
         "end_to_end": {"rows": total_rows, "nnz": e2e_nnz,
@@ -594,7 +594,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, kname, alg,
-                                                 engine.tuning()["stripe_mib"]
+                                                 engine.tuning()["stripes"]
                                                  if kname == "wmh_persistent_kernel" else None),
                      "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms,
                      "timed_span": ("hipGraph replay: descriptor upload, plan, init, %s, "
