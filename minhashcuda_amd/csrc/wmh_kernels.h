@@ -82,7 +82,7 @@ constexpr uint64_t kXcdStripeBytes = 8ull << 20;
 // Stripe count and row threshold for a table of dim x samples parameters:
 // stripes of about `stripe_mib` MiB (at most kMaxStripes of them), rows of at
 // least 1024 nonzeros and of kStripePieceNnz nonzeros per stripe.
-constexpr uint32_t kMaxStripes = 512;
+constexpr uint32_t kMaxStripes = 1024;
 constexpr int kDefaultStripeMib = 4;
 constexpr uint32_t kStripePieceNnz = 8;
 inline void stripe_shape(uint32_t dim, uint32_t samples, int stripe_mib, uint32_t *stripes,
