@@ -77,6 +77,18 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(params, m, samples, target_s, gpu_hashes, what):
     """Times this repository's C oracle of the hashing path on a bounded row
     prefix of `m`: single thread, and OpenMP on up to 16 of the cores this
@@ -113,7 +125,7 @@ def cpu_baseline(params, m, samples, target_s, gpu_hashes, what):
     flips = float(np.mean(np.any(h64 != gpu_hashes[:n3], axis=2)))
     return {
         "value": sub2.nnz * samples / t2, "unit": "nnz*samples/s", "cores": int(used),
-        "kind": "port",
+        "kind": "port", "cpu_model": cpu_model(),
         "sample": "%s: first %d rows (%d nnz), OpenMP C oracle, %.1f s" % (what, n2, sub2.nnz, t2),
         "single_thread_value": sub.nnz * samples / t1,
         "single_thread_sample": "%s: first %d rows (%d nnz), %.1f s" % (what, n1, sub.nnz, t1),
@@ -379,7 +391,8 @@ def c1_bench(args, torch):
                                "GPU value = best of plain / persistent" % m.nnz},
         "gpu": res,
         "cpu_baseline": {"value": m.nnz * SAMPLES / t2, "unit": "nnz*samples/s", "cores": used,
-                         "kind": "port", "sample": "all 6400 rows, OpenMP C oracle",
+                         "kind": "port", "cpu_model": cpu_model(),
+                         "sample": "all 6400 rows, OpenMP C oracle",
                          "single_thread_value": m.nnz * SAMPLES / t1,
                          "single_thread_sample": "all 6400 rows, %.3f s" % t1},
         "bit_exact": bool(np.array_equal(gpu_h, h1) and np.array_equal(h1, h2)),
