@@ -272,7 +272,10 @@ typedef struct {
 } MHCUDAServeStats;
 
 /* Starts the scheduler and completion threads.  The generator must outlive
- * the server; other calls may still use it meanwhile. */
+ * the server.  Hashing calls on the generator may run meanwhile; calls that
+ * change its parameters or tuning (assign / load random vars, set_tuning,
+ * set_stripes, autotune) must wait until mhcuda_serve_stop, since the
+ * server's launches read them without taking the generator's lock. */
 MHCUDAServer *mhcuda_serve_start(MinhashCudaGenerator *gen, const MHCUDAServeConfig *config,
                                  MHCUDAResult *status);
 
