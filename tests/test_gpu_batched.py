@@ -258,7 +258,7 @@ def test_stripes_edge_values(libmhcuda, capi):
     libmhcuda.minhash_cuda_fini(gen)
 
 
-@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("seed", range(12))
 def test_random_batches_and_tunings(libmhcuda, capi, seed):
     """Seeded random batches (row lengths from empty to several thousand,
     sorted or shuffled columns, some columns >= dim, zero weights) under
