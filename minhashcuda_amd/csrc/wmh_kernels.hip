@@ -677,16 +677,20 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
     const uint32_t rel = c - col0;
     const bool fits = __ballot(rel > span) == 0ull;
     const uint32_t o = rel * block_bytes;
-    // 2 kPairSteps divides 64, so a group never reaches past lane 63; lanes
-    // past n hold NaN log-weights
-    // one loop per (division mode, addressing) so that no branch sits
-    // between a group's gathers and its folds
+    // One loop per (division mode, addressing), so that no branch sits
+    // between a group's gathers and its folds.  Full groups of kPairSteps
+    // pairs first, then the chunk's remaining positions one pair at a time
+    // (a piece averages a few dozen nonzeros: padding its last group to a
+    // full one would spend folds and gathers on empty slots).  A pair never
+    // reaches past lane 63 (n is even when it is 64); lanes past n hold NaN
+    // log-weights.
     auto pairs = [&](auto mode, auto near) {
-      for (uint32_t i = 0; i < n; i += 2 * kPairSteps) {
-        float4 r4[kPairSteps], b4[kPairSteps], c4[kPairSteps];
-        float lws[kPairSteps];
+      auto group = [&](auto steps, uint32_t i) {
+        constexpr int P = decltype(steps)::value;
+        float4 r4[P], b4[P], c4[P];
+        float lws[P];
 #pragma unroll
-        for (int u = 0; u < kPairSteps; u++) {
+        for (int u = 0; u < P; u++) {
           const uint32_t src = i + 2u * u + half;
           lws[u] = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
           if constexpr (decltype(near)::value) {
@@ -703,10 +707,14 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
           }
         }
 #pragma unroll
-        for (int u = 0; u < kPairSteps; u++)
+        for (int u = 0; u < P; u++)
           fold4<decltype(mode)::value>(r4[u], b4[u], c4[u], lws[u], cb + i + 2u * u + half,
                                        best, pos);
-      }
+      };
+      uint32_t i = 0;
+      for (; i + 2 * kPairSteps <= n; i += 2 * kPairSteps)
+        group(std::integral_constant<int, kPairSteps>{}, i);
+      for (; i < n; i += 2) group(std::integral_constant<int, 1>{}, i);
     };
     with_division(fast, [&](auto mode) {
       if (fits) pairs(mode, std::true_type{});
