@@ -41,6 +41,8 @@ struct BatchCtl {
   uint32_t invalid;       // nonzero: some indptr decreases; nothing is hashed
   uint32_t seg;           // item size chosen by the plan kernel
   uint32_t n_striped;     // striped rows listed by the init pass
+  unsigned long long total_nnz;    // nonzeros of the batch (plan kernel)
+  unsigned long long striped_nnz;  // nonzeros of the striped rows (init pass)
 };
 
 // Kernel tuning knobs (chosen by the autotuner).

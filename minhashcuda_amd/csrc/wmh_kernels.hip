@@ -50,6 +50,10 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  return ((unsigned long long)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
+}
+
 // ---------------------------------------------------------------------------
 // Parameter generation: Philox4x32-10 (Salmon et al., SC'11) + Gamma(2,1).
 // Replaces the reference's cuRAND + gamma_cuda/log_cuda (kernel.cu:18-36,
@@ -258,18 +262,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const char *blk, ui
                                            kRawBufferWord3);
 }
 
-// Loads V consecutive floats at byte offset off of a block.
-template <int V>
+// Loads V consecutive floats at byte offset off of a block.  Streaming
+// loads carry the non-temporal cache policy (kept out of the way of the L2
+// lines that other wavefronts re-read).
+constexpr int kLoadNonTemporal = 2;  // cache-policy operand: nt
+
+template <int V, bool Streaming = false>
 __device__ __forceinline__ void load_b(__amdgpu_buffer_rsrc_t rs, uint32_t off,
                                        float (&out)[V]) {
+  constexpr int aux = Streaming ? kLoadNonTemporal : 0;
   if constexpr (V == 1) {
-    out[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+    out[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, aux));
   } else if constexpr (V == 2) {
-    const float2 x = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+    const float2 x = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, aux));
     out[0] = x.x;
     out[1] = x.y;
   } else {
-    const float4 x = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+    const float4 x = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, aux));
     out[0] = x.x;
     out[1] = x.y;
     out[2] = x.z;
@@ -477,14 +486,16 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
 // for it; for a stripe piece (`piece_end` != 0) [a, b) lies inside `row`,
 // whose positions are [piece_start, piece_end), and no row bound is read.
 // Unless skip_nnz is 0, rows with at least `skip_nnz` nonzeros are skipped
-// (their stripe pieces hash them); skip_nnz >= 64.
+// (their stripe pieces hash them); skip_nnz >= 64.  `streaming` gathers with
+// non-temporal loads (items hashed while other wavefronts work through a
+// cache-resident stripe).
 template <int V, int U>
 __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint32_t dim,
                                           uint32_t sp, uint32_t samples, uint32_t k0,
                                           LaneOffsets lo, const MatrixDesc &M,
                                           uint32_t a, uint32_t b, uint32_t row,
                                           uint32_t piece_start, uint32_t piece_end,
-                                          uint32_t skip_nnz, bool fast_div) {
+                                          uint32_t skip_nnz, bool fast_div, bool streaming) {
   static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
   const uint32_t lane = lane_id();
   const char *pbase = reinterpret_cast<const char *>(params);
@@ -546,8 +557,9 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
     // column WMH_SENTINEL and are never selected); only real positions can
     // end a row.  A striped row starting inside the chunk runs past its end
     // (it has at least 64 nonzeros): its slots are not folded.
-    with_division(fast, [&](auto mode) {
+    auto groups = [&](auto mode, auto stream) {
       constexpr bool kFast = decltype(mode)::value;
+      constexpr bool kStream = decltype(stream)::value;
       for (uint32_t i = 0; i < n && !skip; i += U) {
         uint32_t j[U];
         float lwu[U];
@@ -558,9 +570,9 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
           lwu[u] = readlane_f32(lw, i + u);
           const __amdgpu_buffer_rsrc_t rs = block_rsrc(
               pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
-          load_b<V>(rs, lo.r, rr[u]);
-          load_b<V>(rs, lo.beta, beta[u]);
-          load_b<V>(rs, lo.lc, lc[u]);
+          load_b<V, kStream>(rs, lo.r, rr[u]);
+          load_b<V, kStream>(rs, lo.beta, beta[u]);
+          load_b<V, kStream>(rs, lo.lc, lc[u]);
         }
         const uint32_t p0 = cb + i;
         if (row_end > p0 + min((uint32_t)U, n - i)) {
@@ -582,6 +594,16 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
           }
         }
       }
+    };
+    with_division(fast, [&](auto mode) {
+      // (only the 2-samples-per-lane kernel, which hashes stripe pieces in
+      // pairs, streams: a second copy of this loop slowed the others)
+      if constexpr (V == 2) {
+        if (streaming) groups(mode, std::true_type{});
+        else groups(mode, std::false_type{});
+      } else {
+        groups(mode, std::false_type{});
+      }
     });
     cb += n;
   }
@@ -602,6 +624,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
 // a chunk holding a column too far from the stripe for that (an unsorted row
 // in a table of 4 GiB or more) uses 64-bit addresses instead.
 constexpr int kPairSteps = 4;  // pairs of nonzeros whose gathers are issued together
+constexpr uint32_t kItemsFirstEvery = 8;
 
 __device__ __forceinline__ uint32_t bperm_u32(uint32_t v, uint32_t src_lane) {
   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
@@ -826,8 +849,23 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   const uint64_t item_units = uniform(ctl->total_units);
   const uint32_t skip_nnz = K ? stripes.min_nnz : 0u;
   const uint32_t home = blockIdx.x % Q;
+  // At 2 samples per lane, when striped rows hold most of the batch (3/4 of
+  // its nonzeros or more), one workgroup in kItemsFirstEvery (spread over the
+  // XCDs) takes the items of the unstriped rows before its stripe queue, with
+  // non-temporal gathers: their random gathers, bound by HBM, then run beside
+  // the striped ones, bound by the texture path and the VALU, without
+  // evicting the stripes from L2 (+1.5% on the uneven batch; with ordinary
+  // loads they cost 2%, one in 2 or 16 workgroups gained less).  Batches of
+  // mostly unstriped rows keep ordinary loads everywhere (non-temporal random
+  // gathers are slower).
+  const bool items_first =
+      V == 2 && K != 0 &&
+      uniform_u64(ctl->striped_nnz) * 4ull >= uniform_u64(ctl->total_nnz) * 3ull &&
+      (blockIdx.x / kXcds) % kItemsFirstEvery == 0;
   for (uint32_t step = 0; step <= Q; step++) {
-    const uint32_t qid = step == 0 ? home : (step == 1 ? kXcds : (home + step - 1) % Q);
+    const uint32_t qid = step == 0 ? (items_first ? kXcds : home)
+                                   : (step == 1 ? (items_first ? home : kXcds)
+                                                : (home + step - 1) % Q);
     const uint32_t nk = qid < Q && qid < K ? (K - qid + Q - 1) / Q : 0u;
     const uint64_t qtotal = qid == kXcds ? item_units : per_stripe * nk;
     if (qtotal == 0) continue;
@@ -893,7 +931,8 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
         const uint32_t koff = (k0 < samples) ? k0 : 0u;
         const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
         hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row, ps, pe,
-                                         row == ~0u ? skip_nnz : 0u, fast_div != 0u);
+                                         row == ~0u ? skip_nnz : 0u, fast_div != 0u,
+                                         items_first && step == 0);
       }
     }
   }
@@ -964,6 +1003,8 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
   if (tid <= kXcds) ctl->next[tid] = 0;
   if (tid == 0) {
     ctl->n_striped = 0;
+    ctl->total_nnz = total;
+    ctl->striped_nnz = 0;
     ctl->invalid = bad;
     ctl->seg = seg;
     ctl->total_items = carry;
@@ -997,20 +1038,29 @@ __global__ __launch_bounds__(kBlock) void batch_rows_kernel(
     const uint32_t r0 = (wv - M.chunk_begin) * kWave;
     const uint32_t row = r0 + lane;
     bool flag = false, striped = false;
+    uint32_t len = 0;  // nonzeros of a striped row
     if (row < M.length) {
       const uint32_t rows0 = M.rows[0];
       const uint32_t s = M.rows[row], e = M.rows[row + 1];
       if (PASS == kInit && (e < s || s < rows0)) atomicOr(&ctl->invalid, 1u);
       const bool empty = s == e;
       striped = e > s && e - s >= stripe_nnz;
+      len = striped ? e - s : 0u;
       const bool split = !empty && (striped || (s - rows0) / seg != (e - 1u - rows0) / seg);
       flag = PASS == kInit ? (empty || split) : split;
     }
     if (PASS == kInit) {
       const unsigned long long smask = __ballot(striped);
       if (smask) {
+        // the wave's striped nonzeros, summed over the ballot's lanes
+        unsigned long long snnz = 0;
+        for (unsigned long long mm = smask; mm; mm &= mm - 1)
+          snnz += readlane_u32(len, (uint32_t)__builtin_ctzll(mm));
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ctl->n_striped, (uint32_t)__builtin_popcountll(smask));
+        if (lane == 0) {
+          base = atomicAdd(&ctl->n_striped, (uint32_t)__builtin_popcountll(smask));
+          atomicAdd(&ctl->striped_nnz, snnz);
+        }
         base = readlane_u32(base, 0);
         if (striped) {
           const uint32_t rank = (uint32_t)__builtin_popcountll(smask & ((1ull << lane) - 1ull));
