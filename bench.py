@@ -474,7 +474,7 @@ def large_bench(args, torch):
         "metric": METRIC, "workload": "large", "unit": "nnz*samples/s",
         "value": nnz * SAMPLES * steps / wall, "steps": steps, "warmup": 1,
         "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "n_gpus": 1,
-        "dtype": "f32", "data": "synthetic", "vs_baseline": None,
+        "scaling": "weak", "dtype": "f32", "data": "synthetic", "vs_baseline": None,
         "config": dict({"workload": "large: 1,000,000 x 12,000,000 CSR, 16,800 nnz/row, "
                                     "uniform(0,100] weights, 128 samples, parameters resident "
                                     "(18.4 GB); value = the autotuned hashing kernel on one "
