@@ -623,7 +623,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
 // a buffer resource at the stripe's first column block with 32-bit offsets;
 // a chunk holding a column too far from the stripe for that (an unsorted row
 // in a table of 4 GiB or more) uses 64-bit addresses instead.
-constexpr int kPairSteps = 4;  // pairs of nonzeros whose gathers are issued together
+constexpr int kPairSteps = 3;  // pairs of nonzeros whose gathers are issued together
 constexpr uint32_t kItemsFirstEvery = 8;
 
 __device__ __forceinline__ uint32_t bperm_u32(uint32_t v, uint32_t src_lane) {
