@@ -45,6 +45,17 @@ def test_logf_exhaustive_correctly_rounded():
     assert r["max_ulp"] <= 0.5 + 1e-6
 
 
+def test_oracle_clean_under_sanitizers():
+    """The C oracle under AddressSanitizer and UndefinedBehaviorSanitizer on
+    edge inputs (oracle/sanitize_check.c): no invalid access, no undefined
+    operation, OpenMP walk equal to the single-thread walk."""
+    oracle_dir = os.path.dirname(oracle.LOGF_CHECK)
+    res = subprocess.run(["make", "-s", "-C", oracle_dir, "sanitize"], capture_output=True,
+                         text=True, timeout=600)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "sanitize_check: ok" in res.stdout
+
+
 def test_logf_specials():
     x = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -1.0, -1e-40, 1.0, 2.0, 1e-45],
                  dtype=np.float32)
