@@ -36,7 +36,10 @@ class Server:
         self._lock = threading.Lock()
 
     def submit(self, m):
-        """Queues a scipy CSR matrix; returns its ticket."""
+        """Queues a scipy CSR matrix; returns its ticket.  The C ABI's indptr
+        is uint32, so a request holds fewer than 2^32 nonzeros."""
+        if m.nnz > 0xFFFFFFFF:
+            raise ValueError("a request must hold fewer than 2^32 nonzeros (uint32 indptr)")
         w = np.ascontiguousarray(m.data, dtype=np.float32)
         c = np.ascontiguousarray(m.indices, dtype=np.uint32)
         r = np.ascontiguousarray(m.indptr, dtype=np.uint32)
