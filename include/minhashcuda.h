@@ -111,8 +111,10 @@ enum MHCUDAKernel {
 };
 
 /* Hashes device-resident CSR data.  `stream` is a hipStream_t; NULL means the
- * generator's own stream and the call returns after completion, otherwise the
- * work is only enqueued on `stream`. */
+ * generator's own stream, ordered after the work already issued to the
+ * legacy default stream, and the call returns after completion; otherwise
+ * the work is only enqueued on `stream`.  (The same holds for every `stream`
+ * argument below.) */
 MHCUDAResult mhcuda_calc_device(const MinhashCudaGenerator *gen,
                                 const float *weights, const uint32_t *cols,
                                 const uint32_t *rows, uint32_t length,
@@ -198,6 +200,14 @@ MHCUDAResult mhcuda_set_timing(MinhashCudaGenerator *gen, int enable);
  * for them to finish); mhcudaInvalidArguments if that call was not timed. */
 MHCUDAResult mhcuda_kernel_time(const MinhashCudaGenerator *gen, float *ms);
 
+/* Test hook: arms a failure of the host-memory paths (mhcuda_calc,
+ * mhcuda_calc_batch, serving) so that callers can check how errors
+ * propagate.  what = 1: the nth device-buffer reservation from now fails
+ * (mhcudaMemoryAllocationFailure); what = 2: the nth host<->device copy from
+ * now fails (mhcudaMemoryCopyError); what = 0 disarms both.  A failed call
+ * leaves nothing in flight, and the generator (or server) stays usable. */
+MHCUDAResult mhcuda_inject_fault(MinhashCudaGenerator *gen, int what, uint32_t nth);
+
 /* ------------------------------------------------------------------------ */
 /* Parameter files: the three tables (or, for tables generated on the device,
  * just the seed) in a versioned, checksummed file, so that a later process
@@ -256,14 +266,15 @@ MHCUDAResult mhcuda_lsh_band_keys(const MinhashCudaGenerator *gen, const uint32_
 typedef struct MHCUDAServer_ MHCUDAServer;
 
 typedef struct {
-  uint64_t memory_budget;       /* device bytes for requests in flight, split evenly over slots */
+  uint64_t memory_budget;       /* device bytes for requests in flight: one pool, shared by
+                                   every slot (a request of up to the whole budget is admitted) */
   uint32_t slots;               /* launches in flight at once (one stream each) */
   uint32_t max_batch_requests;  /* most requests coalesced into one launch */
 } MHCUDAServeConfig;
 
 typedef struct {
   uint64_t completed;           /* requests hashed since the last reset */
-  uint64_t rejected;            /* requests larger than a slot's memory share */
+  uint64_t rejected;            /* requests larger than the whole memory budget */
   uint64_t batches;             /* launches */
   double mean_requests_per_batch;
   double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> hashes in host memory */
@@ -271,17 +282,20 @@ typedef struct {
   double nnz_samples_per_sec;   /* completed nonzeros * samples / window_s */
 } MHCUDAServeStats;
 
-/* Starts the scheduler and completion threads.  The generator must outlive
- * the server.  Hashing calls on the generator may run meanwhile; calls that
+/* Starts the scheduler, slot workers and their device pool (memory_budget
+ * bytes, allocated here; at most 64 slots).  The generator must outlive the
+ * server.  Hashing calls on the generator may run meanwhile; calls that
  * change its parameters or tuning (assign / load random vars, set_tuning,
- * set_stripes, autotune) must wait until mhcuda_serve_stop, since the
- * server's launches read them without taking the generator's lock. */
+ * set_stripes, autotune) return mhcudaInvalidArguments until
+ * mhcuda_serve_stop, since the server's launches read them without taking
+ * the generator's lock. */
 MHCUDAServer *mhcuda_serve_start(MinhashCudaGenerator *gen, const MHCUDAServeConfig *config,
                                  MHCUDAResult *status);
 
 /* Queues one CSR request (host memory, same conventions as mhcuda_calc);
  * the arrays must stay valid and `output` untouched until the request is
- * waited for.  A request that can never fit a slot completes at once with
+ * waited for.  Requests are admitted in arrival order as slots and pool
+ * memory free up; one larger than the whole budget completes at once with
  * mhcudaMemoryAllocationFailure. */
 MHCUDAResult mhcuda_serve_submit(MHCUDAServer *server, const float *weights,
                                  const uint32_t *cols, const uint32_t *rows,

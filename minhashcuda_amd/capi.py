@@ -27,7 +27,7 @@ EXPORTS = (
     "mhcuda_lsh_band_keys", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
     "mhcuda_params_file_info", "mhcuda_set_graph",
     "mhcuda_graph_captures",
-    "mhcuda_set_timing", "mhcuda_kernel_time",
+    "mhcuda_set_timing", "mhcuda_kernel_time", "mhcuda_inject_fault",
     "mhcuda_serve_start", "mhcuda_serve_submit", "mhcuda_serve_wait", "mhcuda_serve_stats",
     "mhcuda_serve_stop",
 )
@@ -129,6 +129,8 @@ def load(path):
     L.mhcuda_set_timing.argtypes = [vp, i32]
     L.mhcuda_kernel_time.restype = i32
     L.mhcuda_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.mhcuda_inject_fault.restype = i32
+    L.mhcuda_inject_fault.argtypes = [vp, i32, u32]
     L.mhcuda_serve_start.restype = vp
     L.mhcuda_serve_start.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(i32)]
     L.mhcuda_serve_submit.restype = i32

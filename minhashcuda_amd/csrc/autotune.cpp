@@ -49,9 +49,11 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
       return mhcudaInvalidArguments;
   }
   std::lock_guard<std::mutex> lock(gen->mu);
+  if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tuning
   const int verbosity = gen->verbosity;
   if (hipSetDevice(gen->device) != hipSuccess) return mhcudaNoSuchDevice;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  if (!stream && wmh::follow_default_stream(gen) != hipSuccess) return mhcudaRuntimeError;
 
   // descriptors (with the 64-row chunk prefix the row scans use) + histogram
   std::vector<wmh::MatrixDesc> descs(count);

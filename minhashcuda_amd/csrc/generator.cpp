@@ -68,6 +68,7 @@ hipError_t DevBuf::reserve(size_t bytes) {
 using wmh::DevBuf;
 
 MinhashCudaGenerator_::~MinhashCudaGenerator_() {
+  if (after_null) (void)hipEventDestroy(after_null);
   if (ev_begin) (void)hipEventDestroy(ev_begin);
   if (ev_end) (void)hipEventDestroy(ev_end);
   if (device >= 0) (void)hipSetDevice(device);
@@ -165,9 +166,12 @@ static PlanLayout plan_layout(const MinhashCudaGenerator *gen,
 }
 
 static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                                    hipStream_t s, PlanCtx *ctx, bool capturing) {
+                                    hipStream_t s, PlanCtx *ctx, bool capturing, bool timed) {
   int verbosity = gen->verbosity;
   const uint32_t count = (uint32_t)mats.size();
+  // the plan kernel keeps the item count below 2^31 + count (it enlarges the
+  // items of huge batches), which must fit the 32-bit item indices
+  if (count > (1u << 30)) return mhcudaInvalidArguments;
   uint64_t chunks = 0;
   for (MatrixDesc &m : mats) {
     m.item_begin = m.n_items = 0;
@@ -183,8 +187,14 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   if (!capturing) {
     if (!ctx->copied)
       HIPCH(hipEventCreateWithFlags(&ctx->copied, hipEventDisableTiming), mhcudaRuntimeError);
+    if (!ctx->released)
+      HIPCH(hipEventCreateWithFlags(&ctx->released, hipEventDisableTiming), mhcudaRuntimeError);
     HIPCH(hipEventSynchronize(ctx->copied), mhcudaRuntimeError);
   }
+  // a grown plan buffer is a new allocation: the old one is freed only after
+  // the launches reading it have finished
+  if (layout.bytes > ctx->plan.cap && !capturing)
+    HIPCH(hipEventSynchronize(ctx->released), mhcudaRuntimeError);
   HIPCH(ctx->plan.reserve(layout.bytes), mhcudaMemoryAllocationFailure);
   if (ctx->cap < count) {
     if (ctx->desc_host) (void)hipHostFree(ctx->desc_host);
@@ -196,17 +206,22 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   }
   std::memcpy(ctx->desc_host, mats.data(), desc_bytes);
   char *dev = (char *)ctx->plan.ptr;
+  // the previous launch through this staging (maybe on another stream) must
+  // be done with the descriptors, queue heads and striped-row list
+  if (!capturing) HIPCH(hipStreamWaitEvent(s, ctx->released, 0), mhcudaRuntimeError);
   HIPCH(hipMemcpyAsync(dev, ctx->desc_host, desc_bytes, hipMemcpyHostToDevice, s),
         mhcudaMemoryCopyError);
   if (!capturing) HIPCH(hipEventRecord(ctx->copied, s), mhcudaRuntimeError);
+  const bool with_events = timed && gen->timing && !capturing;
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
                           (uint32_t)chunks, (BatchCtl *)(dev + ctl_off),
                           (StripedRow *)(dev + layout.list_off), gen->tuning, gen->seg_auto,
                           gen->num_cus, gen->fast_div, s,
-                          gen->timing && !capturing ? gen->ev_begin : nullptr,
-                          gen->timing && !capturing ? gen->ev_end : nullptr),
+                          with_events ? gen->ev_begin : nullptr,
+                          with_events ? gen->ev_end : nullptr),
         mhcudaRuntimeError);
-  gen->timed = gen->timing;
+  if (!capturing) HIPCH(hipEventRecord(ctx->released, s), mhcudaRuntimeError);
+  if (timed) gen->timed = gen->timing;
   return mhcudaSuccess;
 }
 
@@ -216,11 +231,16 @@ PlanCtx::~PlanCtx() {
     (void)hipEventSynchronize(copied);
     (void)hipEventDestroy(copied);
   }
+  if (released) {
+    (void)hipEventSynchronize(released);
+    (void)hipEventDestroy(released);
+  }
   if (desc_host) (void)hipHostFree(desc_host);
 }
 
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx, bool capturing) {
+                               int kernel, hipStream_t s, PlanCtx *ctx, bool capturing,
+                               bool timed) {
   int verbosity = gen->verbosity;
   if (gen->samples % 2 != 0) {
     // the reference finds no even sample_delta for odd counts
@@ -230,16 +250,17 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
   if (kernel == mhcudaKernelAuto) kernel = gen->kernel;
   if (kernel == mhcudaKernelAuto) kernel = mhcudaKernelPersistent;
   if (kernel == mhcudaKernelPersistent)
-    return hash_persistent(gen, mats, s, ctx, capturing);
+    return hash_persistent(gen, mats, s, ctx, capturing, timed);
   if (kernel != mhcudaKernelPlain) return mhcudaInvalidArguments;
-  if (gen->timing && !capturing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
+  const bool with_events = timed && gen->timing && !capturing;
+  if (with_events) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
   for (const MatrixDesc &m : mats) {
     if (m.length == 0) continue;
     HIPCH(launch_plain(gen->params, gen->dim, gen->samples, m, gen->tuning.spl, gen->fast_div, s),
           mhcudaRuntimeError);
   }
-  if (gen->timing && !capturing) HIPCH(hipEventRecord(gen->ev_end, s), mhcudaRuntimeError);
-  gen->timed = gen->timing;
+  if (with_events) HIPCH(hipEventRecord(gen->ev_end, s), mhcudaRuntimeError);
+  if (timed) gen->timed = gen->timing;
   return mhcudaSuccess;
 }
 
@@ -312,6 +333,9 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
     gc.fast_div = gen->fast_div;
     gc.captures++;
   }
+  // replays share the graph's descriptor staging and control block: one on
+  // another stream waits for the previous one to finish
+  HIPCH(hipStreamWaitEvent(s, gc.launched, 0), mhcudaRuntimeError);
   // timing events stay outside the graph: with graphs the timed span is the
   // whole replayed sequence
   if (gen->timing) HIPCH(hipEventRecord(gen->ev_begin, s), mhcudaRuntimeError);
@@ -373,7 +397,8 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
   gen->num_cus = props.multiProcessorCount;
   if (hipStreamCreateWithFlags(&gen->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&gen->copy_stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&gen->d2h_stream, hipStreamNonBlocking) != hipSuccess)
+      hipStreamCreateWithFlags(&gen->d2h_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&gen->after_null, hipEventDisableTiming) != hipSuccess)
     FAIL(mhcudaRuntimeError);
   for (auto &p : gen->pieces) {
     if (p.create() != hipSuccess) FAIL(mhcudaRuntimeError);
@@ -452,6 +477,7 @@ MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *cgen, const f
   if (!cgen || !rs || !ln_cs || !betas) return mhcudaInvalidArguments;
   auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
   std::lock_guard<std::mutex> lock(gen->mu);
+  if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tables
   int verbosity = gen->verbosity;
   HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
   const uint32_t chunk = relayout_chunk(gen->dim, gen->samples);
@@ -514,6 +540,7 @@ MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t
   int verbosity = gen->verbosity;
   HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  if (!stream) HIPCH(wmh::follow_default_stream(gen), mhcudaRuntimeError);
   std::vector<wmh::MatrixDesc> descs(count);
   for (uint32_t m = 0; m < count; m++) {
     descs[m] = wmh::MatrixDesc{mats[m].weights, mats[m].cols, mats[m].rows, mats[m].output,
@@ -527,29 +554,15 @@ MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t
   return mhcudaSuccess;
 }
 
-MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
-                               const MHCUDAMatrix *mats, int kernel) {
-  if (!cgen || !mats || count == 0) return mhcudaInvalidArguments;
-  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+}  // extern "C"
+
+namespace {
+
+// The pipelined host-memory path of mhcuda_calc_batch (called with the
+// generator's lock held and its device current).
+MHCUDAResult calc_batch_locked(MinhashCudaGenerator *gen, uint32_t count,
+                               const MHCUDAMatrix *mats, int kernel, uint64_t max_row) {
   int verbosity = gen->verbosity;
-  uint64_t total_nnz = 0, total_rows = 0, max_row = 0;
-  for (uint32_t m = 0; m < count; m++) {
-    const MHCUDAMatrix &x = mats[m];
-    if (x.length == 0) continue;
-    if (!x.weights || !x.cols || !x.rows || !x.output) return mhcudaInvalidArguments;
-    for (uint32_t r = 0; r < x.length; r++) {
-      if (x.rows[r + 1] < x.rows[r]) return mhcudaInvalidArguments;  // malformed indptr
-      max_row = std::max<uint64_t>(max_row, x.rows[r + 1] - x.rows[r]);
-    }
-    total_nnz += x.rows[x.length] - x.rows[0];
-    total_rows += x.length;
-  }
-  if (total_rows == 0) return mhcudaInvalidArguments;
-  if (gen->samples % 2 != 0) return mhcudaInvalidArguments;
-  std::lock_guard<std::mutex> lock(gen->mu);
-  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
-  DEBUG("mhcuda_calc: %u matrices, %" PRIu64 " rows, %" PRIu64 " nonzeros\n", count,
-        total_rows, total_nnz);
   // Pieces: consecutive row ranges holding at most `budget` nonzeros and
   // `row_budget` rows (a row is never cut).  Piece p uses device buffer set
   // p % 2, so the upload of piece p + 1 overlaps the hashing of piece p and
@@ -601,10 +614,12 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
       nrows += sp.r1 - sp.r0;
     }
     const size_t nspans = pieces[p].size();
-    HIPCH(buf.weights.reserve(std::max<uint64_t>(nnz, 1) * 4), mhcudaMemoryAllocationFailure);
-    HIPCH(buf.cols.reserve(std::max<uint64_t>(nnz, 1) * 4), mhcudaMemoryAllocationFailure);
-    HIPCH(buf.rows.reserve((nrows + nspans) * 4), mhcudaMemoryAllocationFailure);
-    HIPCH(buf.hashes.reserve(std::max<uint64_t>(nrows * S2, 1) * 4),
+    HIPCH(wmh::reserve_checked(gen, buf.weights, std::max<uint64_t>(nnz, 1) * 4),
+          mhcudaMemoryAllocationFailure);
+    HIPCH(wmh::reserve_checked(gen, buf.cols, std::max<uint64_t>(nnz, 1) * 4),
+          mhcudaMemoryAllocationFailure);
+    HIPCH(wmh::reserve_checked(gen, buf.rows, (nrows + nspans) * 4), mhcudaMemoryAllocationFailure);
+    HIPCH(wmh::reserve_checked(gen, buf.hashes, std::max<uint64_t>(nrows * S2, 1) * 4),
           mhcudaMemoryAllocationFailure);
     float *dw = (float *)buf.weights.ptr;
     uint32_t *dc = (uint32_t *)buf.cols.ptr, *dr = (uint32_t *)buf.rows.ptr,
@@ -618,13 +633,16 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
       const uint64_t n = x.rows[sp.r1] - first;
       const uint32_t len = sp.r1 - sp.r0;
       if (n) {
-        HIPCH(hipMemcpyAsync(dw + nnz_off, x.weights + first, n * 4, hipMemcpyHostToDevice,
-                             gen->copy_stream), mhcudaMemoryCopyError);
-        HIPCH(hipMemcpyAsync(dc + nnz_off, x.cols + first, n * 4, hipMemcpyHostToDevice,
-                             gen->copy_stream), mhcudaMemoryCopyError);
+        HIPCH(wmh::copy_checked(gen, dw + nnz_off, x.weights + first, n * 4,
+                                hipMemcpyHostToDevice, gen->copy_stream),
+              mhcudaMemoryCopyError);
+        HIPCH(wmh::copy_checked(gen, dc + nnz_off, x.cols + first, n * 4,
+                                hipMemcpyHostToDevice, gen->copy_stream),
+              mhcudaMemoryCopyError);
       }
-      HIPCH(hipMemcpyAsync(dr + row_off, x.rows + sp.r0, ((uint64_t)len + 1) * 4,
-                           hipMemcpyHostToDevice, gen->copy_stream), mhcudaMemoryCopyError);
+      HIPCH(wmh::copy_checked(gen, dr + row_off, x.rows + sp.r0, ((uint64_t)len + 1) * 4,
+                              hipMemcpyHostToDevice, gen->copy_stream),
+            mhcudaMemoryCopyError);
       dev[i] = wmh::MatrixDesc{dw + nnz_off, dc + nnz_off, dr + row_off, dh + out_off,
                                len, first, 0u, 0u, 0u, 0u};
       nnz_off += n;
@@ -658,8 +676,9 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
         buf.owed.push_back(wmh::PendingCopy{dst, target, bytes});
         land += bytes;
       }
-      HIPCH(hipMemcpyAsync(target, dev[i].output, bytes, hipMemcpyDeviceToHost,
-                           gen->d2h_stream), mhcudaMemoryCopyError);
+      HIPCH(wmh::copy_checked(gen, target, dev[i].output, bytes, hipMemcpyDeviceToHost,
+                              gen->d2h_stream),
+            mhcudaMemoryCopyError);
     }
     HIPCH(hipEventRecord(buf.done, gen->d2h_stream), mhcudaRuntimeError);
   }
@@ -668,6 +687,46 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
   for (auto &b : gen->pieces) b.settle();
   INFO("mhcuda - success\n");
   return mhcudaSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
+                               const MHCUDAMatrix *mats, int kernel) {
+  if (!cgen || !mats || count == 0) return mhcudaInvalidArguments;
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  int verbosity = gen->verbosity;
+  uint64_t total_nnz = 0, total_rows = 0, max_row = 0;
+  for (uint32_t m = 0; m < count; m++) {
+    const MHCUDAMatrix &x = mats[m];
+    if (x.length == 0) continue;
+    if (!x.weights || !x.cols || !x.rows || !x.output) return mhcudaInvalidArguments;
+    for (uint32_t r = 0; r < x.length; r++) {
+      if (x.rows[r + 1] < x.rows[r]) return mhcudaInvalidArguments;  // malformed indptr
+      max_row = std::max<uint64_t>(max_row, x.rows[r + 1] - x.rows[r]);
+    }
+    total_nnz += x.rows[x.length] - x.rows[0];
+    total_rows += x.length;
+  }
+  if (total_rows == 0) return mhcudaInvalidArguments;
+  if (gen->samples % 2 != 0) return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  DEBUG("mhcuda_calc: %u matrices, %" PRIu64 " rows, %" PRIu64 " nonzeros\n", count,
+        total_rows, total_nnz);
+  const MHCUDAResult r = calc_batch_locked(gen, count, mats, kernel, max_row);
+  if (r != mhcudaSuccess) {
+    // a call that failed midway may have left uploads, hashing and downloads
+    // in flight on the generator's streams: drain them, so that the next call
+    // reuses the buffers only once nothing reads or writes them any more
+    (void)hipStreamSynchronize(gen->copy_stream);
+    (void)hipStreamSynchronize(gen->stream);
+    (void)hipStreamSynchronize(gen->d2h_stream);
+    for (auto &b : gen->pieces) b.owed.clear();
+  }
+  return r;
 }
 
 MHCUDAResult mhcuda_set_tuning(MinhashCudaGenerator *gen, int samples_per_lane,
@@ -679,6 +738,7 @@ MHCUDAResult mhcuda_set_tuning(MinhashCudaGenerator *gen, int samples_per_lane,
       waves_per_cu > 32)
     return mhcudaInvalidArguments;
   std::lock_guard<std::mutex> lock(gen->mu);
+  if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tuning
   if (samples_per_lane) gen->tuning.spl = samples_per_lane;
   if (segment > 0) {
     gen->tuning.seg = segment;
@@ -723,6 +783,8 @@ MHCUDAResult mhcuda_jaccard_pairs(const MinhashCudaGenerator *gen, const uint32_
   int verbosity = gen->verbosity;
   HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  if (!stream)
+    HIPCH(wmh::follow_default_stream(const_cast<MinhashCudaGenerator *>(gen)), mhcudaRuntimeError);
   HIPCH(wmh::launch_jaccard_pairs(hashes_a, hashes_b, pairs, n, gen->samples, out, s),
         mhcudaRuntimeError);
   if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
@@ -737,6 +799,8 @@ MHCUDAResult mhcuda_lsh_band_keys(const MinhashCudaGenerator *gen, const uint32_
   int verbosity = gen->verbosity;
   HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  if (!stream)
+    HIPCH(wmh::follow_default_stream(const_cast<MinhashCudaGenerator *>(gen)), mhcudaRuntimeError);
   HIPCH(wmh::launch_band_keys(hashes, rows, gen->samples, bands,
                               reinterpret_cast<unsigned long long *>(keys), s),
         mhcudaRuntimeError);
@@ -769,8 +833,27 @@ MHCUDAResult mhcuda_get_tuning(const MinhashCudaGenerator *gen, int *samples_per
 MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib) {
   if (!gen || stripe_mib < 0) return mhcudaInvalidArguments;
   std::lock_guard<std::mutex> lock(gen->mu);
+  if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tuning
   gen->tuning.stripe_mib = stripe_mib;
   return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_inject_fault(MinhashCudaGenerator *gen, int what, uint32_t nth) {
+  if (!gen) return mhcudaInvalidArguments;
+  switch (what) {
+    case 0:
+      gen->fail_alloc.store(0);
+      gen->fail_copy.store(0);
+      return mhcudaSuccess;
+    case 1:
+      gen->fail_alloc.store(nth);
+      return mhcudaSuccess;
+    case 2:
+      gen->fail_copy.store(nth);
+      return mhcudaSuccess;
+    default:
+      return mhcudaInvalidArguments;
+  }
 }
 
 MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib,

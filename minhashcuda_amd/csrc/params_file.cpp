@@ -293,6 +293,7 @@ MHCUDAResult mhcuda_load_random_vars(const MinhashCudaGenerator *cgen, const cha
   if (!cgen || !path) return mhcudaInvalidArguments;
   auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
   std::lock_guard<std::mutex> lock(gen->mu);
+  if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tables
   const int verbosity = gen->verbosity;
   Fd f;
   f.fd = open(path, O_RDONLY | O_CLOEXEC);

@@ -174,6 +174,26 @@ bool load_csr(PyObject *matrix, Csr *out) {
                     "csr_matrix has 2^32 or more nonzeros: hash it with minhash_cuda_calc");
     return false;
   }
+  // the positions must be non-decreasing and inside data and indices, checked
+  // on their int64 values (a negative position could otherwise wrap into range)
+  {
+    Ref ip64(PyArray_FROM_OTF(indptr.get(), NPY_INT64, NPY_ARRAY_IN_ARRAY | NPY_ARRAY_FORCECAST));
+    if (!ip64) {
+      PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.indptr to numpy.array");
+      return false;
+    }
+    const npy_intp n = PyArray_SIZE(ip64.arr());
+    const int64_t *ip = static_cast<const int64_t *>(PyArray_DATA(ip64.arr()));
+    const int64_t avail =
+        std::min<int64_t>(PyArray_SIZE(out->weights.arr()), PyArray_SIZE(out->cols.arr()));
+    bool ok = n >= 1 && ip[0] >= 0 && ip[n - 1] <= avail;
+    for (npy_intp r = 0; ok && r + 1 < n; r++) ok = ip[r + 1] >= ip[r];
+    if (n >= 1 && !ok) {
+      PyErr_SetString(PyExc_ValueError,
+                      "csr_matrix.indptr must be non-decreasing and within data and indices");
+      return false;
+    }
+  }
   out->rows = as_u32(indptr.get());
   if (!out->rows) {
     PyErr_SetString(PyExc_ValueError, "Failed to convert csr_matrix.indptr to numpy.array");
@@ -182,6 +202,10 @@ bool load_csr(PyObject *matrix, Csr *out) {
   const npy_intp n = PyArray_SIZE(out->rows.arr());
   if (n < 1) {
     PyErr_SetString(PyExc_ValueError, "csr_matrix.indptr is empty");
+    return false;
+  }
+  if (n - 1 > 0xffffffffLL) {
+    PyErr_SetString(PyExc_ValueError, "csr_matrix has 2^32 or more rows");
     return false;
   }
   out->length = static_cast<uint32_t>(n - 1);
@@ -410,13 +434,15 @@ PyObject *py_calc_batch(PyObject *, PyObject *args, PyObject *kwargs) {
   if (!fast) return nullptr;
   const Py_ssize_t count = PySequence_Fast_GET_SIZE(fast.get());
   if (count == 0) return PyList_New(0);
-  const MinhashCudaGeneratorParameters p = mhcuda_get_parameters(gen);
+  // every matrix is checked before the generator is used
   std::vector<Csr> csrs(count);
+  for (Py_ssize_t i = 0; i < count; i++)
+    if (!load_csr(PySequence_Fast_GET_ITEM(fast.get(), i), &csrs[i])) return nullptr;
+  const MinhashCudaGeneratorParameters p = mhcuda_get_parameters(gen);
   std::vector<MHCUDAMatrix> mats(count);
   Ref outs(PyList_New(count));
   if (!outs) return nullptr;
   for (Py_ssize_t i = 0; i < count; i++) {
-    if (!load_csr(PySequence_Fast_GET_ITEM(fast.get(), i), &csrs[i])) return nullptr;
     PyObject *o = new_hashes(csrs[i].length, p.samples);
     if (!o) return nullptr;
     PyList_SET_ITEM(outs.get(), i, o);

@@ -2,16 +2,18 @@
 // reference, which only has the synchronous mhcuda_calc).
 //
 // Callers submit CSR requests held in host memory and receive tickets.  A
-// scheduler thread admits pending requests (FIFO) into a free slot up to the
-// slot's share of the device memory budget, uploads them, and enqueues one
-// persistent hashing launch for the whole group plus the download of every
-// output on the slot's stream.  Each slot has a worker thread that stages
-// the uploads (pinned arrays go straight to DMA), waits for its launch and
-// copies the hashes out, so slots overlap both on the device and on the
-// host; a request that arrives while others are being hashed is admitted as
-// soon as a slot frees up.  Each request's latency is submit -> hashes in
-// host memory.  Each hash equals what a lone mhcuda_calc of that request
-// gives: rows are hashed independently of whatever shares the launch.
+// scheduler thread admits pending requests (FIFO) into a free slot while
+// their device bytes fit one contiguous free range of the server's device
+// pool (the whole memory budget, allocated once), uploads them, and enqueues
+// one persistent hashing launch for the whole group plus the download of
+// every output on the slot's stream.  Each slot has a worker thread that
+// stages the uploads (pinned arrays go straight to DMA, pageable ones through
+// a fixed ring of pinned blocks), waits for its launch and copies the hashes
+// out, so slots overlap both on the device and on the host; a request that
+// arrives while others are being hashed is admitted as soon as a slot and
+// pool memory free up.  Each request's latency is submit -> hashes in host
+// memory.  Each hash equals what a lone mhcuda_calc of that request gives:
+// rows are hashed independently of whatever shares the launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -48,35 +51,159 @@ struct Request {
   double latency_ms = 0;
 };
 
-// One launch in flight: its own stream, device buffer, descriptor staging,
-// pinned landing zones and worker thread.  The worker uploads the admitted
-// requests, enqueues their hashing and downloads, waits for them and hands
-// the hashes back, so host-side copies of different slots run in parallel
-// while their launches overlap on the device.
+uint64_t align256(uint64_t x) { return (x + 255u) & ~(uint64_t)255u; }
+
+// First-fit allocator over the server's device pool (offsets, 256-byte
+// granules), with neighbouring free ranges merged on release.
+class PoolRanges {
+ public:
+  void reset(uint64_t bytes) {
+    free_.clear();
+    if (bytes) free_[0] = bytes;
+  }
+  uint64_t largest() const {
+    uint64_t m = 0;
+    for (const auto &f : free_) m = std::max(m, f.second);
+    return m;
+  }
+  // offset of a range of `bytes`, or ~0 when no free range is large enough
+  uint64_t take(uint64_t bytes) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const uint64_t off = it->first, rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + bytes] = rest;
+      return off;
+    }
+    return ~0ull;
+  }
+  void give(uint64_t off, uint64_t bytes) {
+    if (!bytes) return;
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<uint64_t, uint64_t> free_;
+};
+
+// A ring of pinned host blocks through which a slot stages pageable inputs
+// and outputs: allocated once at server start (no pinned allocation, and no
+// implicit device synchronization of freeing pinned memory, on the request
+// path).  Consecutive small arrays share a block.  All copies through the
+// ring run on the slot's stream; block b's event is recorded after the last
+// copy into or out of it, and the host touches the block again only once
+// that event has completed.
+constexpr size_t kStageBlock = 8u << 20;
+constexpr int kStageBlocks = 4;
+
+struct StageRing {
+  wmh::PinnedBuf mem;
+  hipEvent_t ev[kStageBlocks] = {};
+  int next = 0;      // block taken after the current one
+  int cur = -1;      // block being filled (-1: none)
+  size_t fill = 0;   // bytes used in the current block
+  // landing zones of downloads not yet copied out, per block
+  struct Landing {
+    size_t at;
+    char *dst;
+    size_t bytes;
+  };
+  std::vector<Landing> owed[kStageBlocks];
+
+  bool create() {
+    if (!mem.reserve(kStageBlock * kStageBlocks)) return false;
+    for (auto &e : ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+    return true;
+  }
+  void destroy() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  char *block(int b) { return mem.ptr + (size_t)b * kStageBlock; }
+  // copies out the downloads landed in block b (its event must be complete)
+  void settle(int b) {
+    for (const Landing &l : owed[b]) std::memcpy(l.dst, block(b) + l.at, l.bytes);
+    owed[b].clear();
+  }
+  // closes the current block: its event marks the end of its copies
+  bool close(hipStream_t s) {
+    if (cur < 0) return true;
+    const int b = cur;
+    cur = -1;
+    return hipEventRecord(ev[b], s) == hipSuccess;
+  }
+  // Room for up to `want` bytes in the current block (opening the next block
+  // when it is full); sets *part to the bytes granted.
+  char *room(size_t want, size_t *part, hipStream_t s) {
+    if (cur < 0 || fill == kStageBlock) {
+      if (!close(s)) return nullptr;
+      const int b = next;
+      next = (b + 1) % kStageBlocks;
+      if (hipEventSynchronize(ev[b]) != hipSuccess) return nullptr;
+      settle(b);
+      cur = b;
+      fill = 0;
+    }
+    *part = std::min(want, kStageBlock - fill);
+    char *p = block(cur) + fill;
+    fill = std::min(kStageBlock, (fill + *part + 63) & ~(size_t)63);
+    return p;
+  }
+  // waits for every block and copies out all landed downloads
+  bool drain(hipStream_t s) {
+    if (!close(s)) return false;
+    for (int b = 0; b < kStageBlocks; b++) {
+      if (hipEventSynchronize(ev[b]) != hipSuccess) return false;
+      settle(b);
+    }
+    return true;
+  }
+  void reset() {
+    cur = -1;
+    for (auto &o : owed) o.clear();
+  }
+};
+
+// One launch in flight: its own stream, descriptor staging, pinned staging
+// ring and worker thread.  The worker uploads the admitted requests (into its
+// range of the server's device pool), enqueues their hashing and downloads,
+// waits for them and hands the hashes back, so host-side copies of different
+// slots run in parallel while their launches overlap on the device.
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t finished = nullptr;
-  wmh::DevBuf buffer;
   wmh::PlanCtx plan;
-  wmh::PinnedBuf host_in;    // staging of pageable inputs
-  wmh::PinnedBuf host_out;   // landing zone of hashes bound for pageable outputs
+  StageRing ring;
+  uint64_t pool_off = 0, pool_bytes = 0;  // this group's range of the pool
   std::vector<Request *> reqs;
   bool busy = false;      // requests assigned and not yet retired
   std::thread worker;
   std::condition_variable cv;
 };
 
-uint64_t align256(uint64_t x) { return (x + 255u) & ~(uint64_t)255u; }
-
 }  // namespace
 
 struct MHCUDAServer_ {
   MinhashCudaGenerator *gen = nullptr;
   MHCUDAServeConfig cfg{};
-  uint64_t slot_bytes = 0;
+  wmh::DevBuf pool;        // the memory budget, shared by every slot
+  PoolRanges ranges;       // its free ranges (guarded by mu)
   std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
-  std::condition_variable cv_work;   // scheduler: pending request or free slot
+  std::condition_variable cv_work;   // scheduler: pending request, free slot or freed memory
   std::condition_variable cv_done;   // waiters: a request finished
   std::deque<Request *> pending;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
@@ -95,83 +222,90 @@ struct MHCUDAServer_ {
            align256((uint64_t)length * gen->samples * 8);
   }
 
+  // Uploads `n` bytes from host `src` to device `dst` on the slot's stream:
+  // pinned sources directly, pageable ones through the staging ring.
+  bool upload(Slot *slot, char *dst, const char *src, size_t n, bool pinned) {
+    if (pinned) return wmh::copy_checked(gen, dst, src, n, hipMemcpyHostToDevice,
+                                         slot->stream) == hipSuccess;
+    for (size_t done = 0; done < n;) {
+      size_t part = 0;
+      char *stage = slot->ring.room(n - done, &part, slot->stream);
+      if (!stage) return false;
+      std::memcpy(stage, src + done, part);
+      if (wmh::copy_checked(gen, dst + done, stage, part, hipMemcpyHostToDevice,
+                            slot->stream) != hipSuccess)
+        return false;
+      done += part;
+    }
+    return true;
+  }
+
+  // Downloads `n` bytes of hashes from device `src` to host `dst`: pinned
+  // destinations directly, pageable ones into the staging ring, copied out
+  // when their block is reused or the group drains.
+  bool download(Slot *slot, char *dst, const char *src, size_t n, bool pinned) {
+    if (pinned) return wmh::copy_checked(gen, dst, src, n, hipMemcpyDeviceToHost,
+                                         slot->stream) == hipSuccess;
+    StageRing &ring = slot->ring;
+    for (size_t done = 0; done < n;) {
+      size_t part = 0;
+      char *stage = ring.room(n - done, &part, slot->stream);
+      if (!stage) return false;
+      if (wmh::copy_checked(gen, stage, src + done, part, hipMemcpyDeviceToHost,
+                            slot->stream) != hipSuccess)
+        return false;
+      ring.owed[ring.cur].push_back(
+          StageRing::Landing{(size_t)(stage - ring.block(ring.cur)), dst + done, part});
+      done += part;
+    }
+    return true;
+  }
+
   // Uploads, hashes and downloads the requests of `slot`; returns when the
   // hashes are in the requests' output arrays.
   MHCUDAResult run(Slot *slot) {
-    char *base = (char *)slot->buffer.ptr;
+    char *base = (char *)pool.ptr + slot->pool_off;
     const uint64_t S8 = (uint64_t)gen->samples * 8;
-    // host staging sizes for pageable inputs and outputs
-    size_t in_bytes = 0, out_bytes = 0;
-    std::vector<char> in_pinned(slot->reqs.size() * 3), out_pinned(slot->reqs.size());
-    for (size_t i = 0; i < slot->reqs.size(); i++) {
-      Request *r = slot->reqs[i];
-      const uint32_t first = r->rows[0];
-      in_pinned[3 * i] = r->nnz == 0 || wmh::is_pinned(r->weights + first);
-      in_pinned[3 * i + 1] = r->nnz == 0 || wmh::is_pinned(r->cols + first);
-      in_pinned[3 * i + 2] = wmh::is_pinned(r->rows);
-      if (!in_pinned[3 * i]) in_bytes += align256(r->nnz * 4);
-      if (!in_pinned[3 * i + 1]) in_bytes += align256(r->nnz * 4);
-      if (!in_pinned[3 * i + 2]) in_bytes += align256(((uint64_t)r->length + 1) * 4);
-      out_pinned[i] = wmh::is_pinned(r->output);
-      if (!out_pinned[i]) out_bytes += align256(r->length * S8);
-    }
-    if (!slot->host_in.reserve(in_bytes) || !slot->host_out.reserve(out_bytes))
-      return mhcudaMemoryAllocationFailure;
-    auto upload = [&](void *dst, const void *src, size_t n, bool pinned, size_t &staged) {
-      if (!pinned) {
-        std::memcpy(slot->host_in.ptr + staged, src, n);
-        src = slot->host_in.ptr + staged;
-        staged += align256(n);
-      }
-      return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, slot->stream) == hipSuccess;
-    };
     uint64_t off = 0;
-    size_t staged = 0;
     std::vector<wmh::MatrixDesc> descs;
     descs.reserve(slot->reqs.size());
-    for (size_t i = 0; i < slot->reqs.size(); i++) {
-      Request *r = slot->reqs[i];
+    for (Request *r : slot->reqs) {
       const uint32_t first = r->rows[0];
-      float *dw = (float *)(base + off);
+      char *dw = base + off;
       off += align256(r->nnz * 4);
-      uint32_t *dc = (uint32_t *)(base + off);
+      char *dc = base + off;
       off += align256(r->nnz * 4);
-      uint32_t *dr = (uint32_t *)(base + off);
+      char *dr = base + off;
       off += align256(((uint64_t)r->length + 1) * 4);
-      uint32_t *dh = (uint32_t *)(base + off);
+      char *dh = base + off;
       off += align256(r->length * S8);
-      if (r->nnz && (!upload(dw, r->weights + first, r->nnz * 4, in_pinned[3 * i], staged) ||
-                     !upload(dc, r->cols + first, r->nnz * 4, in_pinned[3 * i + 1], staged)))
+      if (r->nnz &&
+          (!upload(slot, dw, (const char *)(r->weights + first), r->nnz * 4,
+                   wmh::is_pinned(r->weights + first)) ||
+           !upload(slot, dc, (const char *)(r->cols + first), r->nnz * 4,
+                   wmh::is_pinned(r->cols + first))))
         return mhcudaMemoryCopyError;
-      if (!upload(dr, r->rows, ((uint64_t)r->length + 1) * 4, in_pinned[3 * i + 2], staged))
+      if (!upload(slot, dr, (const char *)r->rows, ((uint64_t)r->length + 1) * 4,
+                  wmh::is_pinned(r->rows)))
         return mhcudaMemoryCopyError;
-      descs.push_back(wmh::MatrixDesc{dw, dc, dr, dh, r->length, first, 0u, 0u, 0u, 0u});
+      descs.push_back(wmh::MatrixDesc{(const float *)dw, (const uint32_t *)dc,
+                                      (const uint32_t *)dr, (uint32_t *)dh, r->length, first,
+                                      0u, 0u, 0u, 0u});
     }
+    // (untimed: the generator's timing state belongs to the calls that hold
+    // its lock; serving keeps its own latency statistics)
     MHCUDAResult res = wmh::hash_device_batch(gen, descs, mhcudaKernelPersistent, slot->stream,
-                                              &slot->plan, false);
+                                              &slot->plan, false, false);
     if (res != mhcudaSuccess) return res;
-    std::vector<size_t> out_off(slot->reqs.size());
-    size_t landed = 0;
     for (size_t i = 0; i < slot->reqs.size(); i++) {
       Request *r = slot->reqs[i];
-      void *dst = r->output;
-      if (!out_pinned[i]) {
-        out_off[i] = landed;
-        dst = slot->host_out.ptr + landed;
-        landed += align256(r->length * S8);
-      }
-      if (hipMemcpyAsync(dst, descs[i].output, r->length * S8, hipMemcpyDeviceToHost,
-                         slot->stream) != hipSuccess)
+      if (!download(slot, (char *)r->output, (const char *)descs[i].output, r->length * S8,
+                    wmh::is_pinned(r->output)))
         return mhcudaMemoryCopyError;
     }
     if (hipEventRecord(slot->finished, slot->stream) != hipSuccess ||
-        hipEventSynchronize(slot->finished) != hipSuccess)
+        hipEventSynchronize(slot->finished) != hipSuccess || !slot->ring.drain(slot->stream))
       return mhcudaRuntimeError;
-    for (size_t i = 0; i < slot->reqs.size(); i++) {
-      if (!out_pinned[i])
-        std::memcpy(slot->reqs[i]->output, slot->host_out.ptr + out_off[i],
-                    slot->reqs[i]->length * S8);
-    }
     return mhcudaSuccess;
   }
 
@@ -183,7 +317,13 @@ struct MHCUDAServer_ {
       if (slot->reqs.empty()) break;  // scheduler finished and nothing assigned
       lock.unlock();
       MHCUDAResult st = run(slot);
-      if (st != mhcudaSuccess) (void)hipStreamSynchronize(slot->stream);
+      // after a failure nothing of this group may still be in flight when its
+      // pool range and staging blocks are reused
+      if (st != mhcudaSuccess) {
+        (void)hipStreamSynchronize(slot->stream);
+        (void)slot->ring.close(slot->stream);
+        slot->ring.reset();
+      }
       const Clock::time_point now = Clock::now();
       lock.lock();
       for (Request *r : slot->reqs) {
@@ -196,6 +336,8 @@ struct MHCUDAServer_ {
         }
       }
       if (now > last_done) last_done = now;
+      ranges.give(slot->pool_off, slot->pool_bytes);
+      slot->pool_bytes = 0;
       slot->reqs.clear();
       slot->busy = false;
       cv_work.notify_one();
@@ -203,32 +345,36 @@ struct MHCUDAServer_ {
     }
   }
 
+  Slot *free_slot() {
+    for (auto &s : slots)
+      if (!s->busy) return s.get();
+    return nullptr;
+  }
+
   void schedule_loop() {
     std::unique_lock<std::mutex> lock(mu);
     for (;;) {
-      Slot *slot = nullptr;
+      // admissible: a free slot, and a free pool range for the oldest request
+      // (every admitted request is at most the budget, so once the slots
+      // drain the whole pool is one free range and it fits)
       cv_work.wait(lock, [&] {
         if (stopping && pending.empty()) return true;
-        if (pending.empty()) return false;
-        for (auto &s : slots)
-          if (!s->busy) return true;
-        return false;
+        return !pending.empty() && free_slot() != nullptr &&
+               pending.front()->bytes <= ranges.largest();
       });
       if (pending.empty()) break;  // stopping and drained
-      for (auto &s : slots) {
-        if (!s->busy) {
-          slot = s.get();
-          break;
-        }
-      }
-      // admission: FIFO while the slot's memory budget and request cap allow
+      Slot *slot = free_slot();
+      // admission: FIFO while the largest free range and the request cap allow
+      const uint64_t room = ranges.largest();
       uint64_t used = 0;
       while (!pending.empty() && slot->reqs.size() < cfg.max_batch_requests &&
-             used + pending.front()->bytes <= slot_bytes) {
+             used + pending.front()->bytes <= room) {
         used += pending.front()->bytes;
         slot->reqs.push_back(pending.front());
         pending.pop_front();
       }
+      slot->pool_off = ranges.take(used);
+      slot->pool_bytes = used;
       slot->busy = true;
       batches++;
       batched_requests += slot->reqs.size();
@@ -260,22 +406,39 @@ MHCUDAServer *mhcuda_serve_start(MinhashCudaGenerator *gen, const MHCUDAServeCon
     if (status) *status = code;
     return nullptr;
   };
-  if (!gen || !config || config->slots == 0 || config->memory_budget == 0 ||
-      config->max_batch_requests == 0)
+  if (!gen || !config || config->slots == 0 || config->slots > 64 ||
+      config->memory_budget == 0 || config->max_batch_requests == 0)
     return fail(mhcudaInvalidArguments);
   if (gen->samples % 2 != 0) return fail(mhcudaInvalidArguments);
   if (hipSetDevice(gen->device) != hipSuccess) return fail(mhcudaNoSuchDevice);
   std::unique_ptr<MHCUDAServer> srv(new MHCUDAServer());
+  auto fail_after = [&](MHCUDAResult code) -> MHCUDAServer * {
+    for (auto &s : srv->slots) {  // no thread was started yet
+      s->ring.destroy();
+      if (s->finished) (void)hipEventDestroy(s->finished);
+      if (s->stream) (void)hipStreamDestroy(s->stream);
+    }
+    return fail(code);
+  };
   srv->gen = gen;
   srv->cfg = *config;
-  srv->slot_bytes = config->memory_budget / config->slots;
+  const uint64_t budget = config->memory_budget & ~(uint64_t)255u;
+  if (budget == 0) return fail(mhcudaInvalidArguments);
+  if (wmh::reserve_checked(gen, srv->pool, budget) != hipSuccess)
+    return fail(mhcudaMemoryAllocationFailure);
+  srv->ranges.reset(budget);
   for (uint32_t i = 0; i < config->slots; i++) {
-    std::unique_ptr<Slot> s(new Slot());
+    srv->slots.emplace_back(new Slot());
+    Slot *s = srv->slots.back().get();
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s->finished, hipEventDisableTiming) != hipSuccess)
-      return fail(mhcudaRuntimeError);
-    if (s->buffer.reserve(srv->slot_bytes) != hipSuccess) return fail(mhcudaMemoryAllocationFailure);
-    srv->slots.push_back(std::move(s));
+      return fail_after(mhcudaRuntimeError);
+    if (!s->ring.create()) return fail_after(mhcudaMemoryAllocationFailure);
+  }
+  {
+    // from here on, calls that change the tables or the tuning are refused
+    std::lock_guard<std::mutex> lock(gen->mu);
+    gen->servers++;
   }
   MHCUDAServer *p = srv.get();
   for (auto &s : srv->slots) {
@@ -311,8 +474,8 @@ MHCUDAResult mhcuda_serve_submit(MHCUDAServer *srv, const float *weights, const 
     srv->first_submit = req->submitted;
     srv->have_window = true;
   }
-  if (req->bytes > srv->slot_bytes) {
-    // larger than a slot's share of the memory budget: never admissible
+  if (req->bytes > (srv->cfg.memory_budget & ~(uint64_t)255u)) {
+    // larger than the whole memory budget: never admissible
     req->status = mhcudaMemoryAllocationFailure;
     req->done = true;
     srv->rejected++;
@@ -379,8 +542,13 @@ MHCUDAResult mhcuda_serve_stop(MHCUDAServer *srv) {
   (void)hipSetDevice(srv->gen->device);
   for (auto &s : srv->slots) {
     (void)hipStreamSynchronize(s->stream);
+    s->ring.destroy();
     (void)hipEventDestroy(s->finished);
     (void)hipStreamDestroy(s->stream);
+  }
+  {
+    std::lock_guard<std::mutex> lock(srv->gen->mu);
+    srv->gen->servers--;
   }
   delete srv;
   return mhcudaSuccess;

@@ -35,8 +35,8 @@ struct MatrixDesc {
 constexpr uint32_t kXcds = 8;
 struct BatchCtl {
   unsigned long long next[kXcds + 1];  // queue heads
-  uint32_t total_units;   // items * sample tiles
-  uint32_t total_items;
+  unsigned long long total_units;      // items * sample tiles
+  uint32_t total_items;   // below 2^31 + count (see batch_plan_kernel)
   uint32_t ntiles;
   uint32_t invalid;       // nonzero: some indptr decreases; nothing is hashed
   uint32_t seg;           // item size chosen by the plan kernel

@@ -814,7 +814,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
       (unsigned long long)min(8u, stripes.per_unit),
       min((unsigned long long)stripes.per_unit, pieces * Q / (4ull * waves)));
   const uint64_t per_stripe = (pieces + per_unit - 1) / per_unit;
-  const uint64_t item_units = uniform(ctl->total_units);
+  const uint64_t item_units = uniform_u64(ctl->total_units);
   const uint32_t skip_nnz = K ? stripes.min_nnz : 0u;
   const uint32_t home = blockIdx.x % Q;
   // At 2 samples per lane, when striped rows hold most of the batch (3/4 of
@@ -910,7 +910,9 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
 // matrices, and the reset of the work queue.  With `target_items` > 0 the
 // item size is the batch's nonzeros / target_items rounded down to a multiple
 // of 64, within [64, max_seg], so that a small batch still yields enough
-// items to occupy every CU; otherwise it is max_seg.
+// items to occupy every CU; otherwise it is max_seg.  Either way it is at
+// least ceil(nonzeros / 2^31) rounded up to 64, so that the item count stays
+// below 2^31 + count and fits the 32-bit item indices.
 __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restrict__ mats,
                                                             uint32_t count, uint32_t max_seg,
                                                             uint32_t target_items,
@@ -941,6 +943,8 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
     const unsigned long long want = (total / target_items) & ~63ull;
     seg = (uint32_t)min((unsigned long long)max_seg, max(64ull, want));
   }
+  const unsigned long long min_seg = (((total + (1ull << 31) - 1) >> 31) + 63ull) & ~63ull;
+  if (seg < min_seg) seg = (uint32_t)min_seg;
   // pass 2: item prefix
   for (uint32_t base = 0; base < count; base += kBlock) {
     const uint32_t m = base + tid;
@@ -976,7 +980,7 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
     ctl->invalid = bad;
     ctl->seg = seg;
     ctl->total_items = carry;
-    ctl->total_units = carry * ntiles;
+    ctl->total_units = (unsigned long long)carry * ntiles;
     ctl->ntiles = ntiles;
   }
 }

@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -98,14 +99,18 @@ struct PieceBuffers {
   }
 };
 
-// Descriptor staging of one stream's persistent launches: pinned host copy,
-// device copy plus control block, and the event marking the end of the last
-// upload out of the pinned copy (so that it is not overwritten too early).
+// Descriptor staging of persistent launches: pinned host copy, device copy
+// plus control block, the event marking the end of the last upload out of
+// the pinned copy (so that the host does not overwrite it too early), and the
+// event marking the end of the last launch that read the device copy (the
+// next call's stream waits for it before overwriting the descriptors, so
+// calls on different streams never share them while in flight).
 struct PlanCtx {
   DevBuf plan;
   MatrixDesc *desc_host = nullptr;
   size_t cap = 0;
   hipEvent_t copied = nullptr;
+  hipEvent_t released = nullptr;
   PlanCtx() = default;
   PlanCtx(const PlanCtx &) = delete;
   PlanCtx &operator=(const PlanCtx &) = delete;
@@ -161,6 +166,9 @@ struct MinhashCudaGenerator_ {
   bool fast_div = false;
   wmh::DevBuf range_flag;
   hipStream_t stream = nullptr;
+  // recorded on the legacy default stream by device calls given no stream,
+  // so that `stream` (non-blocking) waits for work already issued there
+  hipEvent_t after_null = nullptr;
   std::mutex mu;
   // grow-only caches, reference minhashcuda.cc:27-31
   wmh::DevBuf staging;
@@ -184,6 +192,13 @@ struct MinhashCudaGenerator_ {
   bool timing = false;
   bool timed = false;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  // serving loops running on this generator: while any runs, calls that
+  // change the parameters or the tuning are refused (the servers' launches
+  // read them without taking `mu`)
+  std::atomic<int> servers{0};
+  // armed failures (mhcuda_inject_fault): the n-th device allocation / copy
+  // of the host-memory paths from now fails; 0 = disarmed
+  std::atomic<uint32_t> fail_alloc{0}, fail_copy{0};
 
   MinhashCudaGenerator_() = default;
   MinhashCudaGenerator_(const MinhashCudaGenerator_ &) = delete;
@@ -198,8 +213,12 @@ namespace wmh {
 // synchronization of `ctx` with other streams.
 // `capturing`: the stream is being captured into a graph, so nothing may
 // allocate or wait on the host (ctx's buffers must already be reserved).
+// `timed`: record the generator's timing events (when timing is on) and mark
+// the call as the one mhcuda_kernel_time reports; callers that do not hold
+// the generator's lock (the serving loop) pass false.
 MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats,
-                               int kernel, hipStream_t s, PlanCtx *ctx, bool capturing);
+                               int kernel, hipStream_t s, PlanCtx *ctx, bool capturing,
+                               bool timed = true);
 
 // The hipGraph-replayed form of hash_device_batch for device-pointer calls.
 MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats, int kernel,
@@ -208,5 +227,35 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
 // Checks the r table on the device after the parameters change and sets
 // gen->fast_div (blocks until the check is done).
 MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen);
+
+// Device-pointer calls given no stream run on gen->stream, which does not
+// synchronize with the legacy default stream by itself: this makes it wait
+// for everything already issued there (the caller's inputs may be written
+// by work on the default stream, torch's default among them).
+inline hipError_t follow_default_stream(MinhashCudaGenerator *gen) {
+  hipError_t e = hipEventRecord(gen->after_null, nullptr);
+  if (e == hipSuccess) e = hipStreamWaitEvent(gen->stream, gen->after_null, 0);
+  return e;
+}
+
+// Armed failures of the host-memory paths (mhcuda_inject_fault): each call
+// counts one device allocation / copy and returns true for the one that must
+// fail.
+inline bool armed_failure(std::atomic<uint32_t> &left) {
+  uint32_t n = left.load();
+  while (n != 0) {
+    if (left.compare_exchange_weak(n, n - 1)) return n == 1;
+  }
+  return false;
+}
+inline hipError_t reserve_checked(MinhashCudaGenerator *gen, DevBuf &buf, size_t bytes) {
+  if (armed_failure(gen->fail_alloc)) return hipErrorOutOfMemory;
+  return buf.reserve(bytes);
+}
+inline hipError_t copy_checked(MinhashCudaGenerator *gen, void *dst, const void *src,
+                               size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  if (armed_failure(gen->fail_copy)) return hipErrorInvalidValue;
+  return hipMemcpyAsync(dst, src, bytes, kind, s);
+}
 
 }  // namespace wmh

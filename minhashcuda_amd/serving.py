@@ -38,11 +38,18 @@ class Server:
     def submit(self, m):
         """Queues a scipy CSR matrix; returns its ticket.  The C ABI's indptr
         is uint32, so a request holds fewer than 2^32 nonzeros."""
-        if m.nnz > 0xFFFFFFFF:
+        ip = np.asarray(m.indptr, dtype=np.int64)
+        if m.nnz > 0xFFFFFFFF or (len(ip) and ip[-1] > 0xFFFFFFFF):
             raise ValueError("a request must hold fewer than 2^32 nonzeros (uint32 indptr)")
+        if len(ip) < 2:
+            raise ValueError("a request needs at least one row")
+        # the positions are read on the host and uploaded as they are: they
+        # must stay inside data and indices
+        if ip[0] < 0 or np.any(np.diff(ip) < 0) or ip[-1] > min(len(m.data), len(m.indices)):
+            raise ValueError("csr_matrix.indptr must be non-decreasing and within data and indices")
         w = np.ascontiguousarray(m.data, dtype=np.float32)
         c = np.ascontiguousarray(m.indices, dtype=np.uint32)
-        r = np.ascontiguousarray(m.indptr, dtype=np.uint32)
+        r = ip.astype(np.uint32)
         out = np.empty((m.shape[0], self.samples, 2), np.uint32)
         t = ctypes.c_uint64(0)
         capi_mod.check(self.lib.mhcuda_serve_submit(
@@ -58,6 +65,11 @@ class Server:
         arrays are uploaded and downloaded without staging copies."""
         for a in (weights, cols, indptr, output):
             assert a.flags["C_CONTIGUOUS"]
+        if len(indptr) < 2 or output.size != (len(indptr) - 1) * self.samples * 2:
+            raise ValueError("indptr must hold rows + 1 entries and output rows * samples * 2")
+        ip = np.asarray(indptr).view(np.uint32)
+        if np.any(ip[1:] < ip[:-1]) or ip[-1] > min(len(weights), len(cols)):
+            raise ValueError("indptr must be non-decreasing and within weights and cols")
         t = ctypes.c_uint64(0)
         capi_mod.check(self.lib.mhcuda_serve_submit(
             self.handle, weights.ctypes.data, cols.ctypes.data, indptr.ctypes.data,

@@ -64,6 +64,9 @@ def lib():
             L.wmh_oracle_philox.argtypes = [_u32p, _u32p, _u32p]
             L.wmh_oracle_gen_params.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_uint32, _f32p, _f32p, _f32p]
+            L.wmh_oracle_gen_params_cols.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
+                                                     ctypes.c_uint32, _u32p, ctypes.c_uint64,
+                                                     _f32p, _f32p, _f32p]
             _lib = L
     return _lib
 
@@ -140,6 +143,33 @@ def gen_params(seed, dim, samples):
     betas = np.empty_like(rs)
     lib().wmh_oracle_gen_params(seed, dim, samples, rs, ln_cs, betas)
     return rs, ln_cs, betas
+
+
+def gen_params_cols(seed, dim, samples, cols):
+    """Columns `cols` of gen_params(seed, dim, samples), [samples][len(cols)]
+    (only the columns a sample of rows touches of a table too large to copy)."""
+    cols = np.ascontiguousarray(cols, dtype=np.uint32)
+    n = len(cols)
+    rs = np.empty((samples, n), np.float32)
+    ln_cs = np.empty_like(rs)
+    betas = np.empty_like(rs)
+    lib().wmh_oracle_gen_params_cols(seed, dim, samples, cols, n, rs, ln_cs, betas)
+    return rs, ln_cs, betas
+
+
+def calc_touched(seed, dim, samples, m, threads=16):
+    """Hashes of CSR matrix `m` under the generated table gen_params(seed, dim,
+    samples), computing only the table columns m touches: its columns are
+    renumbered in increasing order (which keeps every nonzero's parameters
+    and the CSR order, so every hash's arithmetic is unchanged) and mapped
+    back in the output."""
+    idx = np.asarray(m.indices, dtype=np.int64)
+    uniq, local = np.unique(idx, return_inverse=True)
+    params = gen_params_cols(seed, dim, samples, uniq)
+    h, _ = calc_omp(*params, m.data, local.astype(np.uint32), m.indptr, threads)
+    sel = h[..., 0] != SENTINEL
+    h[..., 0] = np.where(sel, uniq.astype(np.uint32)[np.where(sel, h[..., 0], 0)], h[..., 0])
+    return h
 
 
 # --------------------------------------------------------------------------

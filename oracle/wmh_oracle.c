@@ -213,19 +213,37 @@ static float u_half_open(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
 //   ln_cs =  ln(-ln(u(A2) * u(A3)))
 //   betas =  u'(B0)
 // Outputs are in the reference layout [samples][dim].
+// One parameter triple of the generated table: sample k of column j of a
+// table `dim` columns wide (counter k * dim + j).
+static void gen_one(uint32_t seed, uint64_t n, float *r, float *lc, float *beta) {
+  const uint32_t key[2] = {seed, WMH_PHILOX_KEY1};
+  uint32_t a[4], b[4];
+  uint32_t ca[4] = {(uint32_t)n, (uint32_t)(n >> 32), 0u, 0u};
+  uint32_t cb[4] = {(uint32_t)n, (uint32_t)(n >> 32), 1u, 0u};
+  wmh_oracle_philox(ca, key, a);
+  wmh_oracle_philox(cb, key, b);
+  *r = -wmh_logf(u_open(a[0]) * u_open(a[1]));
+  *lc = wmh_logf(-wmh_logf(u_open(a[2]) * u_open(a[3])));
+  *beta = u_half_open(b[0]);
+}
+
 void wmh_oracle_gen_params(uint32_t seed, uint32_t dim, uint32_t samples,
                            float *rs, float *ln_cs, float *betas) {
-  const uint32_t key[2] = {seed, WMH_PHILOX_KEY1};
   const uint64_t total = (uint64_t)dim * samples;
 #pragma omp parallel for schedule(static)
-  for (int64_t n = 0; n < (int64_t)total; n++) {
-    uint32_t a[4], b[4];
-    uint32_t ca[4] = {(uint32_t)n, (uint32_t)((uint64_t)n >> 32), 0u, 0u};
-    uint32_t cb[4] = {(uint32_t)n, (uint32_t)((uint64_t)n >> 32), 1u, 0u};
-    wmh_oracle_philox(ca, key, a);
-    wmh_oracle_philox(cb, key, b);
-    rs[n] = -wmh_logf(u_open(a[0]) * u_open(a[1]));
-    ln_cs[n] = wmh_logf(-wmh_logf(u_open(a[2]) * u_open(a[3])));
-    betas[n] = u_half_open(b[0]);
+  for (int64_t n = 0; n < (int64_t)total; n++) gen_one(seed, (uint64_t)n, rs + n, ln_cs + n, betas + n);
+}
+
+// The columns cols[0..ncols) of the table wmh_oracle_gen_params would produce
+// for `dim` columns, as [samples][ncols] tables (column i of the result is
+// table column cols[i]): the counter-based generator needs no other column.
+void wmh_oracle_gen_params_cols(uint32_t seed, uint32_t dim, uint32_t samples,
+                                const uint32_t *cols, uint64_t ncols, float *rs,
+                                float *ln_cs, float *betas) {
+  const uint64_t total = ncols * samples;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)total; i++) {
+    const uint64_t k = (uint64_t)i / ncols, c = (uint64_t)i % ncols;
+    gen_one(seed, k * dim + cols[c], rs + i, ln_cs + i, betas + i);
   }
 }

@@ -101,3 +101,31 @@ def test_python_argument_errors(libmhcuda):
     with pytest.raises(ValueError):
         libmhcuda.minhash_cuda_assign_vars(0, [1.0], [1.0], [1.0])
     assert libmhcuda.minhash_cuda_fini(0) is None
+
+
+def _corrupt_indptr_matrices():
+    """CSR matrices whose indptr was changed after construction so that it
+    points past data / indices, decreases, or starts negative."""
+    import scipy.sparse as sp
+    out = []
+    m = sp.csr_matrix(np.eye(4, dtype=np.float32))
+    m.indptr[-1] = 9          # past len(data) == 4
+    out.append(m)
+    m = sp.csr_matrix(np.eye(4, dtype=np.float32))
+    m.indptr[2] = 0           # decreasing
+    out.append(m)
+    m = sp.csr_matrix(np.eye(4, dtype=np.float32))
+    m.indptr = m.indptr.astype(np.int64)
+    m.indptr[0] = -(1 << 32)  # wraps to 0 as uint32
+    out.append(m)
+    return out
+
+
+def test_python_calc_entry_points_check_indptr_before_any_device_work(libmhcuda):
+    # the checks run before the generator is touched: a pointer that is never
+    # dereferenced stands in for one (no GPU on this host)
+    for m in _corrupt_indptr_matrices():
+        with pytest.raises(ValueError, match="indptr"):
+            libmhcuda.minhash_cuda_calc(1, m)
+        with pytest.raises(ValueError, match="indptr"):
+            libmhcuda.minhash_cuda_calc_batch(1, [m])

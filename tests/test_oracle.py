@@ -297,3 +297,20 @@ def test_lsh_candidate_pairs():
     big = np.zeros((100, 1), np.uint64)
     assert len(lsh.candidate_pairs(big, max_bucket=64)) == 0
     assert len(lsh.candidate_pairs(big[:10], max_bucket=64)) == 45
+
+
+def test_touched_columns_oracle_equals_full_table():
+    """calc_touched (the generated table's columns a matrix touches, renumbered
+    in order) gives the hashes of calc on the whole generated table, and
+    gen_params_cols gives exactly those columns of gen_params."""
+    dim, samples = 7000, 64
+    full = oracle.gen_params(11, dim, samples)
+    cols = np.array([6999, 0, 17, 3500, 17], np.uint32)
+    part = oracle.gen_params_cols(11, dim, samples, cols)
+    for a, b in zip(part, full):
+        assert np.array_equal(a, b[:, cols])
+    rng = np.random.default_rng(3)
+    m = sp.random(80, dim, density=0.02, format="csr", random_state=4, dtype=np.float32)
+    m.data[:] = rng.lognormal(0, 1.5, m.nnz).astype(np.float32)
+    m.data[::7] = 0.0  # rows with unselectable entries, some with none at all
+    assert np.array_equal(oracle.calc_touched(11, dim, samples, m), oracle.calc_csr(*full, m))

@@ -92,3 +92,24 @@ def test_requests_beyond_uint32_indptr_are_refused():
     with pytest.raises(ValueError, match="2\\^32"):
         srv.submit(huge)
     assert not lib.requests
+
+
+def test_requests_with_indptr_outside_the_arrays_are_refused():
+    lib = StubLib()
+    srv = serving.Server(1, lib=lib)
+    bad = []
+    m = sp.csr_matrix(np.eye(4, dtype=np.float32))
+    m.indptr[-1] = 9          # past data and indices
+    bad.append(m)
+    m = sp.csr_matrix(np.eye(4, dtype=np.float32))
+    m.indptr[2] = 0           # decreasing
+    bad.append(m)
+    for m in bad:
+        with pytest.raises(ValueError, match="indptr"):
+            srv.submit(m)
+    w = np.ones(4, np.float32)
+    c = np.zeros(4, np.uint32)
+    out = np.empty((2, StubLib.SAMPLES, 2), np.uint32)
+    with pytest.raises(ValueError, match="indptr"):
+        srv.submit_arrays(w, c, np.array([0, 3, 7], np.uint32), out)
+    assert not lib.requests
