@@ -120,6 +120,21 @@ MHCUDAResult mhcuda_calc_device(const MinhashCudaGenerator *gen,
                                 const uint32_t *rows, uint32_t length,
                                 uint32_t *output, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device or
+ * host memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same as mhcuda_calc_device with the persistent
+ * kernel, but the launch plan lives in a fixed device scratch array of
+ * 8 MiB + 512 bytes per GPU (looked up at mhcuda_init, kept until the process
+ * ends) instead of the generator's grow-only buffers: nothing is allocated.
+ * At most 1,048,576 rows per call (more: mhcudaInvalidArguments).  Every call
+ * on one GPU shares that GPU's scratch, so calls that overlap on one GPU (on
+ * different streams, or from generators on the same device) race: order them
+ * on one stream or synchronize between them. */
+MHCUDAResult mhcuda_calc_device_noalloc(const MinhashCudaGenerator *gen,
+                                        const float *weights, const uint32_t *cols,
+                                        const uint32_t *rows, uint32_t length,
+                                        uint32_t *output, void *stream);
+
 /* Hashes `count` device-resident matrices with the chosen kernel (plain: one
  * launch per matrix; persistent: one launch for the whole batch). */
 MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *gen,

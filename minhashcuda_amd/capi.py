@@ -21,7 +21,7 @@ KERNEL_AUTO, KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1, 2
 EXPORTS = (
     "mhcuda_init", "mhcuda_get_parameters", "mhcuda_retrieve_random_vars",
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
-    "mhcuda_calc_device", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
+    "mhcuda_calc_device", "mhcuda_calc_device_noalloc", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
     "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_stripes", "mhcuda_get_stripes",
     "mhcuda_autotune", "mhcuda_jaccard_pairs",
     "mhcuda_lsh_band_keys", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
@@ -94,6 +94,8 @@ def load(path):
     L.mhcuda_fini.argtypes = [vp]
     L.mhcuda_calc_device.restype = i32
     L.mhcuda_calc_device.argtypes = [vp, vp, vp, vp, u32, vp, vp]
+    L.mhcuda_calc_device_noalloc.restype = i32
+    L.mhcuda_calc_device_noalloc.argtypes = [vp, vp, vp, vp, u32, vp, vp]
     L.mhcuda_calc_batch_device.restype = i32
     L.mhcuda_calc_batch_device.argtypes = [vp, u32, ctypes.POINTER(Matrix), i32, vp]
     L.mhcuda_calc_batch.restype = i32

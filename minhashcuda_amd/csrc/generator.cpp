@@ -385,6 +385,8 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
     return nullptr;                  \
   } while (false)
   if (hipSetDevice(dev) != hipSuccess) FAIL(mhcudaNoSuchDevice);
+  // the scratch of mhcuda_calc_device_noalloc, looked up once per device
+  if (wmh::scratch_init(dev) != hipSuccess) FAIL(mhcudaRuntimeError);
   std::unique_ptr<MinhashCudaGenerator> gen(new MinhashCudaGenerator());
   gen->dim = dim;
   gen->samples = samples;
@@ -526,6 +528,36 @@ MHCUDAResult mhcuda_calc_device(const MinhashCudaGenerator *cgen, const float *w
     return mhcudaInvalidArguments;
   MHCUDAMatrix m{weights, cols, rows, length, output};
   return mhcuda_calc_batch_device(cgen, 1, &m, mhcudaKernelAuto, stream);
+}
+
+MHCUDAResult mhcuda_calc_device_noalloc(const MinhashCudaGenerator *cgen, const float *weights,
+                                        const uint32_t *cols, const uint32_t *rows,
+                                        uint32_t length, uint32_t *output, void *stream) {
+  if (!cgen || !weights || !cols || !rows || !output || length == 0)
+    return mhcudaInvalidArguments;
+  if (length > wmh::kNoAllocMaxRows) return mhcudaInvalidArguments;  // beyond the scratch
+  auto *gen = const_cast<MinhashCudaGenerator *>(cgen);
+  if (gen->samples % 2 != 0) return mhcudaInvalidArguments;  // as mhcuda_calc_device
+  std::lock_guard<std::mutex> lock(gen->mu);
+  int verbosity = gen->verbosity;
+  HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
+  char *scr = static_cast<char *>(wmh::scratch_address(gen->device));
+  if (!scr) return mhcudaRuntimeError;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : gen->stream;
+  if (!stream) HIPCH(wmh::follow_default_stream(gen), mhcudaRuntimeError);
+  // the plan kernel resets the queue heads and counters and the init pass
+  // rewrites the striped-row list, so nothing left by an earlier call is read
+  const wmh::MatrixDesc desc{weights, cols, rows, output, length, 0u, 0u, 0u, 0u, 0u};
+  HIPCH(wmh::launch_store_desc(desc, reinterpret_cast<wmh::MatrixDesc *>(scr), s),
+        mhcudaRuntimeError);
+  HIPCH(wmh::launch_persistent(gen->params, gen->dim, gen->samples,
+                               reinterpret_cast<wmh::MatrixDesc *>(scr), 1u, (length + 63u) / 64u,
+                               reinterpret_cast<wmh::BatchCtl *>(scr + wmh::kScratchCtlOff),
+                               reinterpret_cast<wmh::StripedRow *>(scr + wmh::kScratchListOff),
+                               gen->tuning, gen->seg_auto, gen->num_cus, gen->fast_div, s),
+        mhcudaRuntimeError);
+  if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
+  return mhcudaSuccess;
 }
 
 MHCUDAResult mhcuda_calc_batch_device(const MinhashCudaGenerator *cgen, uint32_t count,

@@ -98,6 +98,28 @@ inline void stripe_shape(uint32_t dim, uint32_t samples, int stripe_mib, uint32_
   *min_nnz = *stripes > 1 ? (rows_min > 1024u ? rows_min : 1024u) : 0xffffffffu;
 }
 
+// Device scratch of the allocation-free entry point
+// (mhcuda_calc_device_noalloc): the plan of one persistent launch over one
+// matrix -- its descriptor at offset 0, the control block at kScratchCtlOff
+// and the striped-row list (8 bytes per row) at kScratchListOff -- in a
+// file-scope __device__ array of kScratchBytes, one per GPU, kept until the
+// process ends.
+constexpr uint32_t kNoAllocMaxRows = 1u << 20;
+constexpr size_t kScratchCtlOff = 256;
+constexpr size_t kScratchListOff = 512;
+constexpr size_t kScratchBytes = kScratchListOff + sizeof(StripedRow) * kNoAllocMaxRows;
+static_assert(sizeof(MatrixDesc) <= kScratchCtlOff, "descriptor fits its scratch slot");
+static_assert(kScratchCtlOff + sizeof(BatchCtl) <= kScratchListOff, "control block fits");
+
+// Looks up the scratch array's address on `device` (which must be current)
+// once, with hipGetSymbolAddress; later calls return at once.
+hipError_t scratch_init(int device);
+// The scratch array's device address on `device` (null before scratch_init).
+void *scratch_address(int device);
+// Writes one descriptor into device memory with a one-thread kernel (the
+// descriptor travels as a kernel argument: no copy from host memory).
+hipError_t launch_store_desc(const MatrixDesc &desc, MatrixDesc *dst, hipStream_t stream);
+
 hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
                              uint32_t seed, hipStream_t stream);
 

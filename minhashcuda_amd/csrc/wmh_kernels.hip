@@ -22,6 +22,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "wmh_div.h"
@@ -1157,12 +1158,47 @@ __global__ __launch_bounds__(kBlock) void band_keys_kernel(const uint2 *__restri
   }
 }
 
+// Plan of the allocation-free entry point (see scratch_init).  Host code
+// never uses this name as an address: it takes the per-device address that
+// hipGetSymbolAddress returns.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+__global__ void store_desc_kernel(MatrixDesc desc, MatrixDesc *__restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = desc;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+
 unsigned grid_for(uint64_t waves) {
   const uint64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
   return (unsigned)(blocks < (1u << 20) ? (blocks ? blocks : 1) : (1u << 20));
 }
 
 }  // namespace
+
+hipError_t scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[device]) return hipSuccess;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+  scratch_ptr[device] = p;
+  return hipSuccess;
+}
+
+void *scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_ptr[device];
+}
+
+hipError_t launch_store_desc(const MatrixDesc &desc, MatrixDesc *dst, hipStream_t stream) {
+  hipLaunchKernelGGL(store_desc_kernel, dim3(1), dim3(64), 0, stream, desc, dst);
+  return hipGetLastError();
+}
 
 hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
                              uint32_t seed, hipStream_t stream) {
