@@ -170,6 +170,20 @@ MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib);
 MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib,
                                 uint32_t *stripes, uint32_t *min_nnz);
 
+/* The skip bound of the persistent kernel's column-stripe pieces (on by
+ * default): a 2-byte-per-parameter-pair table of lower bounds of the hash
+ * value lets a piece skip the exact evaluation of every (nonzero, sample)
+ * that provably cannot beat the row's current minimum.  The hashes are the
+ * same either way.  It needs a sample count divisible by 8, tables with
+ * every r in [2^-60, 2^60], beta in [0, 1) and |ln_c| <= 2^60 (generated
+ * tables always qualify; assigned or loaded ones are checked), and
+ * dim * roundup(samples, 4) * 2 bytes of device memory (allocated by
+ * mhcuda_init; MHCUDA_SKIP_BOUND=0 in the environment skips that).
+ * mhcuda_set_skip(0) turns it off; mhcuda_get_skip reports the switch and
+ * whether the bound is in use for the current tables. */
+MHCUDAResult mhcuda_set_skip(MinhashCudaGenerator *gen, int enable);
+MHCUDAResult mhcuda_get_skip(const MinhashCudaGenerator *gen, int *enabled, int *active);
+
 /* Result of mhcuda_autotune. */
 typedef struct {
   uint64_t key;              /* samples | p50 bin << 16 | p90 bin << 24 | size class << 32 */
@@ -301,7 +315,7 @@ typedef struct {
  * bytes, allocated here; at most 64 slots).  The generator must outlive the
  * server.  Hashing calls on the generator may run meanwhile; calls that
  * change its parameters or tuning (assign / load random vars, set_tuning,
- * set_stripes, autotune) return mhcudaInvalidArguments until
+ * set_stripes, set_skip, autotune) return mhcudaInvalidArguments until
  * mhcuda_serve_stop, since the server's launches read them without taking
  * the generator's lock. */
 MHCUDAServer *mhcuda_serve_start(MinhashCudaGenerator *gen, const MHCUDAServeConfig *config,

@@ -23,6 +23,7 @@ EXPORTS = (
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_device_noalloc", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
     "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_stripes", "mhcuda_get_stripes",
+    "mhcuda_set_skip", "mhcuda_get_skip",
     "mhcuda_autotune", "mhcuda_jaccard_pairs",
     "mhcuda_lsh_band_keys", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
     "mhcuda_params_file_info", "mhcuda_set_graph",
@@ -110,6 +111,10 @@ def load(path):
     L.mhcuda_get_stripes.restype = i32
     L.mhcuda_get_stripes.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(u32),
                                      ctypes.POINTER(u32)]
+    L.mhcuda_set_skip.restype = i32
+    L.mhcuda_set_skip.argtypes = [vp, i32]
+    L.mhcuda_get_skip.restype = i32
+    L.mhcuda_get_skip.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.mhcuda_autotune.restype = i32
     L.mhcuda_autotune.argtypes = [vp, u32, ctypes.POINTER(Matrix), vp, ctypes.POINTER(TuneResult)]
     L.mhcuda_jaccard_pairs.restype = i32

@@ -80,6 +80,7 @@ MinhashCudaGenerator_::~MinhashCudaGenerator_() {
   if (copy_stream) (void)hipStreamDestroy(copy_stream);
   if (d2h_stream) (void)hipStreamDestroy(d2h_stream);
   if (params) (void)hipFree(params);
+  if (bounds) (void)hipFree(bounds);
 }
 
 namespace {
@@ -218,7 +219,7 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
                           (StripedRow *)(dev + layout.list_off), gen->tuning, gen->seg_auto,
                           gen->num_cus, gen->fast_div, s,
                           with_events ? gen->ev_begin : nullptr,
-                          with_events ? gen->ev_end : nullptr),
+                          with_events ? gen->ev_end : nullptr, launch_bounds_of(gen)),
         mhcudaRuntimeError);
   if (!capturing) HIPCH(hipEventRecord(ctx->released, s), mhcudaRuntimeError);
   if (timed) gen->timed = gen->timing;
@@ -267,6 +268,7 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
 MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen) {
   int verbosity = gen->verbosity;
   gen->fast_div = false;
+  gen->bounded = false;
   HIPCH(gen->range_flag.reserve(sizeof(uint32_t)), mhcudaMemoryAllocationFailure);
   HIPCH(hipMemsetAsync(gen->range_flag.ptr, 0, sizeof(uint32_t), gen->stream), mhcudaRuntimeError);
   HIPCH(launch_params_range(gen->params, gen->dim, gen->samples, (uint32_t *)gen->range_flag.ptr,
@@ -276,7 +278,13 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen) {
   HIPCH(hipMemcpyAsync(&bad, gen->range_flag.ptr, sizeof(bad), hipMemcpyDeviceToHost, gen->stream),
         mhcudaMemoryCopyError);
   HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
-  gen->fast_div = bad == 0;
+  gen->fast_div = (bad & kRangeDiv) == 0;
+  if (gen->bounds && bad == 0) {
+    HIPCH(launch_build_bounds(gen->params, gen->dim, gen->samples, gen->bounds, gen->stream),
+          mhcudaRuntimeError);
+    HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
+    gen->bounded = true;
+  }
   return mhcudaSuccess;
 }
 
@@ -296,7 +304,8 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
                     gc.tuning.spl == gen->tuning.spl && gc.tuning.seg == gen->tuning.seg &&
                     gc.tuning.waves_per_cu == gen->tuning.waves_per_cu &&
                     gc.tuning.stripe_mib == gen->tuning.stripe_mib &&
-                    gc.fast_div == gen->fast_div && gc.key.size() == mats.size() &&
+                    gc.fast_div == gen->fast_div &&
+                    gc.bounds == launch_bounds_of(gen) && gc.key.size() == mats.size() &&
                     std::memcmp(gc.key.data(), mats.data(), sizeof(MatrixDesc) * mats.size()) == 0;
   if (!same) {
     if (!gc.capture)
@@ -331,6 +340,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
     gc.kernel = kernel;
     gc.tuning = gen->tuning;
     gc.fast_div = gen->fast_div;
+    gc.bounds = launch_bounds_of(gen);
     gc.captures++;
   }
   // replays share the graph's descriptor staging and control block: one on
@@ -420,6 +430,16 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
     INFO("failed to allocate %zu bytes for the parameters\n", param_bytes);
     FAIL(mhcudaMemoryAllocationFailure);
   }
+  // the bound table of the skip bound (a sixth of the parameter bytes); the
+  // engine hashes without it when it cannot be allocated or is switched off
+  const char *skip_env = std::getenv("MHCUDA_SKIP_BOUND");
+  if (!(skip_env && std::strcmp(skip_env, "0") == 0)) {
+    if (hipMalloc((void **)&gen->bounds, (size_t)dim * gen->sp * sizeof(uint16_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      gen->bounds = nullptr;
+      INFO("no memory for the bound table: hashing without the skip bound\n");
+    }
+  }
   if (deferred) {
     if (hipMemsetAsync(gen->params, 0, param_bytes, gen->stream) != hipSuccess)
       FAIL(mhcudaRuntimeError);
@@ -488,6 +508,7 @@ MHCUDAResult mhcuda_assign_random_vars(const MinhashCudaGenerator *cgen, const f
   const float *src[3] = {rs, betas, ln_cs};
   gen->params_source = MinhashCudaGenerator::kParamsNone;  // until every table is in
   gen->fast_div = false;                                   // until the new r are checked
+  gen->bounded = false;                                    // until the bound table is rebuilt
   for (uint32_t which = 0; which < 3; which++) {
     for (uint32_t j0 = 0; j0 < gen->dim; j0 += chunk) {
       const uint32_t n = std::min(chunk, gen->dim - j0);
@@ -554,7 +575,8 @@ MHCUDAResult mhcuda_calc_device_noalloc(const MinhashCudaGenerator *cgen, const 
                                reinterpret_cast<wmh::MatrixDesc *>(scr), 1u, (length + 63u) / 64u,
                                reinterpret_cast<wmh::BatchCtl *>(scr + wmh::kScratchCtlOff),
                                reinterpret_cast<wmh::StripedRow *>(scr + wmh::kScratchListOff),
-                               gen->tuning, gen->seg_auto, gen->num_cus, gen->fast_div, s),
+                               gen->tuning, gen->seg_auto, gen->num_cus, gen->fast_div, s,
+                               nullptr, nullptr, wmh::launch_bounds_of(gen)),
         mhcudaRuntimeError);
   if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
   return mhcudaSuccess;
@@ -867,6 +889,21 @@ MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib) {
   std::lock_guard<std::mutex> lock(gen->mu);
   if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tuning
   gen->tuning.stripe_mib = stripe_mib;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_set_skip(MinhashCudaGenerator *gen, int enable) {
+  if (!gen) return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads it
+  gen->skip = enable != 0;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_get_skip(const MinhashCudaGenerator *gen, int *enabled, int *active) {
+  if (!gen) return mhcudaInvalidArguments;
+  if (enabled) *enabled = gen->skip ? 1 : 0;
+  if (active) *active = wmh::launch_bounds_of(gen) != nullptr && gen->samples % 8 == 0;
   return mhcudaSuccess;
 }
 

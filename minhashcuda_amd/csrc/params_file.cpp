@@ -312,6 +312,7 @@ MHCUDAResult mhcuda_load_random_vars(const MinhashCudaGenerator *cgen, const cha
   if (hipSetDevice(gen->device) != hipSuccess) return mhcudaNoSuchDevice;
   gen->params_source = MinhashCudaGenerator::kParamsNone;  // until every table is in
   gen->fast_div = false;                                   // until the new r are checked
+  gen->bounded = false;                                    // until the bound table is rebuilt
 
   if (h.kind == kKindSeeded) {
     if (wmh::launch_gen_params(gen->params, dim, samples, h.seed, gen->stream) != hipSuccess ||
@@ -378,6 +379,7 @@ MHCUDAResult mhcuda_load_random_vars(const MinhashCudaGenerator *cgen, const cha
     (void)hipMemsetAsync(gen->params, 0, (size_t)dim * 3u * gen->sp * sizeof(float), gen->stream);
     (void)hipStreamSynchronize(gen->stream);
     gen->fast_div = false;
+    gen->bounded = false;
     return r;
   }
   gen->params_source = MinhashCudaGenerator::kParamsTables;

@@ -123,11 +123,18 @@ hipError_t launch_store_desc(const MatrixDesc &desc, MatrixDesc *dst, hipStream_
 hipError_t launch_gen_params(float *params, uint32_t dim, uint32_t samples,
                              uint32_t seed, hipStream_t stream);
 
-// Sets *bad (device memory, zeroed by the caller) to nonzero unless every r of
-// the table (the first `samples` entries of each column's r run) lies in
-// [2^-60, 2^60]; NaNs count as outside.
+// Range checks of the tables: ORs into *bad (device memory, zeroed by the
+// caller) kRangeDiv unless every r of a real sample lies in [2^-60, 2^60]
+// (NaNs count as outside), and kRangeBound unless every beta lies in [0, 1)
+// and every ln_c in [-2^60, 2^60].
+constexpr uint32_t kRangeDiv = 1u, kRangeBound = 2u;
 hipError_t launch_params_range(const float *params, uint32_t dim, uint32_t samples,
                                uint32_t *bad, hipStream_t stream);
+
+// Fills the bound table [dim][sp] (fp16 bits, wmh_bound.h) from the
+// parameter table; valid only while both range checks pass.
+hipError_t launch_build_bounds(const float *params, uint32_t dim, uint32_t samples,
+                               uint16_t *bounds, hipStream_t stream);
 
 // Scatter one [samples][ncols] FP32 table (rows of pitch ncols) into
 // params[(j0 + jj) * 3 * sp + which * sp + k].
@@ -157,11 +164,15 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
 // `striped` (device memory, one entry per row of the batch) receives the
 // striped rows when tuning.stripe_mib asks for stripes; it may be null
 // otherwise.
+// `bounds`: the bound table when the skip bound may be used (both range
+// checks passed, samples a multiple of 8), else null; only the
+// 2-samples-per-lane kernel's stripe pieces use it.
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
                              bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
-                             hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                             hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+                             const uint16_t *bounds = nullptr);
 
 // Weighted-Jaccard estimates of row pairs (pairs[2i], pairs[2i+1] index the
 // rows of hash tables a and b, each [rows][samples][2]).

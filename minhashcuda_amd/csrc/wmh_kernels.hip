@@ -25,6 +25,7 @@
 #include <mutex>
 #include <type_traits>
 
+#include "wmh_bound.h"
 #include "wmh_div.h"
 #include "wmh_kernels.h"
 #include "wmh_math.h"
@@ -157,9 +158,13 @@ __global__ __launch_bounds__(kBlock) void gather_params_kernel(
   }
 }
 
-// Range check of the r table for the short division (div_rn_fast): *bad
-// becomes nonzero when some r of a real sample is outside [2^-60, 2^60] or
-// NaN.  One ballot and at most one atomic per wavefront.
+// Range checks of the tables, one bit each in *bad (one ballot and at most
+// one atomic per wavefront and bit):
+//   kRangeDiv:   some r of a real sample is outside [2^-60, 2^60] or NaN (the
+//                short division div_rn_fast needs it inside);
+//   kRangeBound: some beta is outside [0, 1) or some ln_c is not finite or
+//                beyond 2^60 in magnitude (the skip bound of wmh_bound.h
+//                needs it inside, and r inside the division range).
 __global__ __launch_bounds__(kBlock) void params_range_kernel(const float *__restrict__ params,
                                                               uint32_t dim, uint32_t samples,
                                                               uint32_t sp,
@@ -167,18 +172,40 @@ __global__ __launch_bounds__(kBlock) void params_range_kernel(const float *__res
   const uint64_t total = (uint64_t)dim * samples;
   const uint64_t stride = (uint64_t)gridDim.x * kBlock;
   // every lane of a wavefront runs the same number of iterations, so the
-  // ballot is taken with the whole wavefront present
+  // ballots are taken with the whole wavefront present
   for (uint64_t base = (uint64_t)blockIdx.x * kBlock + (threadIdx.x & ~(uint64_t)(kWave - 1));
        base < total; base += stride) {
     const uint64_t idx = base + lane_id();
-    bool out = false;
+    bool out_div = false, out_bound = false;
     if (idx < total) {
       const uint32_t j = (uint32_t)(idx / samples);
       const uint32_t k = (uint32_t)(idx - (uint64_t)j * samples);
-      const float r = params[(uint64_t)j * 3u * sp + k];
-      out = !(r >= kFastDivMinB && r <= kFastDivMaxB);
+      const float *col = params + (uint64_t)j * 3u * sp;
+      const float r = col[k], beta = col[sp + k], lc = col[2u * sp + k];
+      out_div = !(r >= kFastDivMinB && r <= kFastDivMaxB);
+      out_bound = !(beta >= 0.0f && beta < 1.0f) ||
+                  !(lc >= -WMH_BOUND_MAX_ABS_LC && lc <= WMH_BOUND_MAX_ABS_LC);
     }
-    if (__ballot(out) != 0ull && lane_id() == 0) atomicOr(bad, 1u);
+    const unsigned long long md = __ballot(out_div), mb = __ballot(out_bound);
+    if (lane_id() == 0 && (md | mb) != 0ull)
+      atomicOr(bad, (md ? kRangeDiv : 0u) | (mb ? kRangeBound : 0u));
+  }
+}
+
+// The bound table: entry (j, k) = wmh_bound_entry(r, ln_c) of column j,
+// sample k, laid out [dim][sp] (one 2 * sp-byte run per column); padded
+// samples get +inf (every bound test skips them).
+__global__ __launch_bounds__(kBlock) void build_bounds_kernel(const float *__restrict__ params,
+                                                              uint32_t dim, uint32_t samples,
+                                                              uint32_t sp,
+                                                              uint16_t *__restrict__ bounds) {
+  const uint64_t total = (uint64_t)dim * sp;
+  for (uint64_t idx = (uint64_t)blockIdx.x * kBlock + threadIdx.x; idx < total;
+       idx += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t j = (uint32_t)(idx / sp);
+    const uint32_t k = (uint32_t)(idx - (uint64_t)j * sp);
+    const float *col = params + (uint64_t)j * 3u * sp;
+    bounds[idx] = k < samples ? wmh_bound_entry(col[k], col[2u * sp + k]) : (uint16_t)0x7c00u;
   }
 }
 
@@ -770,6 +797,213 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
   }
 }
 
+// Stripe pieces of a 128-sample tile with the skip bound (wmh_bound.h), for
+// pieces whose row already holds a keyed minimum for every sample of the
+// tile (the earlier stripes' pieces wrote them).  Quarter-wave layout: lane
+// l hashes samples k0 = 128 tile + 8 (l % 16) .. k0 + 7 at the positions
+// cb + 4m + l / 16 (m = 0..15) of each 64-position chunk, so one 16-byte load
+// per lane covers four nonzeros x 128 samples of the 2-byte bound table.
+//   Phase 1: for every (nonzero, sample) the bound L = X - lwb is compared
+//   with the sample's threshold, min(the row's keyed minimum read at the
+//   start, the quarter's own minimum): a lane marks nonzero m when any of its
+//   8 samples has L <= threshold (NaN: marked).  A computed ln_a is never
+//   below L, so an unmarked nonzero is above the row's minimum (or not below
+//   the quarter's, which comes earlier in CSR order) for every sample and
+//   cannot change the merged result.
+//   Phase 2: each lane evaluates its marked nonzeros exactly (8 samples, the
+//   formula of kernel.cu:103-106) in CSR order, one per round.
+// The four quarters merge by (ln_a, position) and lanes 0-15 merge the result
+// into the row's keyed minima like every piece of a split row.  Requires
+// samples % 8 == 0 (every lane's 8 samples lie in its column's runs).
+constexpr int kBoundGroup = 4;  // phase-1 steps whose bound loads are issued together
+
+// The value of a keyed minimum (order_key(ln_a) << 32 | position).
+__device__ __forceinline__ float key_value(unsigned long long key) {
+  const uint32_t h = (uint32_t)(key >> 32);
+  return __uint_as_float((h & 0x80000000u) ? (h & 0x7fffffffu) : ~h);
+}
+
+__device__ __forceinline__ float half_lo(uint32_t x) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(x & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t x) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(x >> 16));
+}
+
+// Folds samples H*4 .. H*4 + 3 of the lane's 8 (one 16-byte load of each
+// parameter array) for the nonzero at position p into the running minima.
+template <bool Fast, int H>
+__device__ __forceinline__ void fold_half(const float4 &r4, const float4 &b4, const float4 &c4,
+                                          float lw, uint32_t p, bool on, float (&best)[8],
+                                          uint32_t (&pos)[8], float (&thr)[8]) {
+  WMH_NO_CONTRACT
+  const float r[4] = {r4.x, r4.y, r4.z, r4.w};
+  const float beta[4] = {b4.x, b4.y, b4.z, b4.w};
+  const float lc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int v = 4 * H + i;
+    const float t = __builtin_floorf(div_lw<Fast>(lw, r[i]) + beta[i]);  // kernel.cu:103
+    const float ln_y = (t - beta[i]) * r[i];              // kernel.cu:104
+    const float ln_a = (lc[i] - ln_y) - r[i];             // kernel.cu:105
+    const bool upd = on && ln_a < best[v];                // kernel.cu:106
+    best[v] = upd ? ln_a : best[v];
+    pos[v] = upd ? p : pos[v];
+    thr[v] = upd && ln_a < thr[v] ? ln_a : thr[v];
+  }
+}
+
+__device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
+                                                   const uint16_t *__restrict__ bounds,
+                                                   uint32_t dim, uint32_t sp, uint32_t samples,
+                                                   uint32_t tile, const MatrixDesc &M, uint32_t a,
+                                                   uint32_t b, uint32_t row, uint32_t col0_in,
+                                                   bool fast_div, const float (&slot_best)[8]) {
+  const uint32_t lane = lane_id();
+  const uint32_t quarter = lane >> 4;
+  const uint32_t col0 = uniform(col0_in);
+  const uint32_t k0 = tile * 128u + 8u * (lane & 15u);
+  const bool lane_ok = k0 < samples;
+  const uint32_t koff = lane_ok ? k0 : 0u;
+  const uint32_t block_bytes = 12u * sp, bound_bytes = 2u * sp;
+  const uint32_t s_b = uniform(sp * 4u), s_c = uniform(sp * 8u);  // beta, ln_c arrays
+  // column offsets c - col0 up to `span` keep every lane's bytes below 2^32
+  const uint32_t span = (0xffffffffu - 12u * sp) / block_bytes;
+  // the resources must live in SGPRs: their bases are made wave-uniform
+  const uint64_t pb = reinterpret_cast<uint64_t>(params) + (uint64_t)col0 * block_bytes;
+  const uint64_t bb = reinterpret_cast<uint64_t>(bounds) + (uint64_t)col0 * bound_bytes;
+  char *pbase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(pb >> 32)) << 32) |
+                                         uniform((uint32_t)pb));
+  char *bbase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(bb >> 32)) << 32) |
+                                         uniform((uint32_t)bb));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(pbase, (short)0, (int)0xffffffff, kRawBufferWord3);
+  const __amdgpu_buffer_rsrc_t rb =
+      __builtin_amdgcn_make_buffer_rsrc(bbase, (short)0, (int)0xffffffff, kRawBufferWord3);
+  const uint32_t *__restrict__ cols = M.cols - M.base;
+  const float *__restrict__ weights = M.weights - M.base;
+  // every gather goes through the two resources with 32-bit offsets; a piece
+  // holding a column too far from the stripe for that (an unsorted row in a
+  // table of 4 GiB or more) is left to hash_piece_pairs
+  // (the offsets are those of the gathers below: lanes without a column
+  // below dim gather from the stripe's first column, or from column 0 when
+  // the stripe starts at dim, which does not fit and leaves the piece to
+  // hash_piece_pairs)
+  for (uint32_t cb = a; cb < b; cb += kWave) {
+    const uint32_t n = uniform(min((uint32_t)kWave, b - cb));
+    uint32_t c = col0 < dim ? col0 : 0u;
+    if (lane < n) {
+      const uint32_t cc = cols[cb + lane];
+      if (cc < dim) c = cc;
+    }
+    if (__ballot(c - col0 > span) != 0ull) return false;
+  }
+  float best[8], thr[8];
+  uint32_t pos[8];
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    best[v] = FLT_MAX;
+    pos[v] = WMH_SENTINEL;
+    thr[v] = slot_best[v];
+  }
+  for (uint32_t cb = a; cb < b; cb += kWave) {
+    const uint32_t n = uniform(min((uint32_t)kWave, b - cb));
+    // lanes without a selectable nonzero gather from a valid block (the
+    // stripe's first, or column 0 for a stripe past dim) and are skipped by
+    // every bound test (lwb = -inf) and never selected (lw = NaN)
+    uint32_t c = col0 < dim ? col0 : 0u;
+    float lw = __builtin_nanf("");
+    float lwb = -__builtin_inff();
+    if (lane < n) {
+      const uint32_t cc = cols[cb + lane];
+      if (cc < dim) {
+        c = cc;
+        lw = wmh_logf(weights[cb + lane]);  // once per nonzero (kernel.cu:95)
+        lwb = wmh_bound_lw(lw);
+      }
+    }
+    const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
+    const uint32_t rel = c - col0;
+    const uint32_t o = rel * block_bytes, ob = rel * bound_bytes;
+    const uint32_t steps = uniform((n + 3u) >> 2);
+    // phase 1: the lane's marked nonzeros, bit m for position cb + 4m + quarter
+    uint32_t marked = 0u;
+    {
+      for (uint32_t m0 = 0; m0 < steps; m0 += kBoundGroup) {
+        uint4 x[kBoundGroup];
+        float lb[kBoundGroup];
+#pragma unroll
+        for (int u = 0; u < kBoundGroup; u++) {
+          const uint32_t src = 4u * (m0 + u) + quarter;  // lanes past n (or 63) read valid data
+          lb[u] = __uint_as_float(bperm_u32(__float_as_uint(lwb), src));
+          x[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                               rb, bperm_u32(ob, src) + 2u * koff, 0, 0));
+        }
+#pragma unroll
+        for (int u = 0; u < kBoundGroup; u++) {
+          const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+          bool hit = false;
+#pragma unroll
+          for (int h = 0; h < 4; h++) {
+            hit |= !(half_lo(w[h]) - lb[u] > thr[2 * h]);
+            hit |= !(half_hi(w[h]) - lb[u] > thr[2 * h + 1]);
+          }
+          const uint32_t m = m0 + u;
+          if (hit && m < steps && 4u * m + quarter < n && lane_ok) marked |= 1u << m;
+        }
+      }
+    }
+    // phase 2: exact evaluation of the marked nonzeros, lowest position first
+    auto phase2 = [&](auto mode) {
+      for (;;) {
+        const bool has = marked != 0u;
+        if (__ballot(has) == 0ull) break;
+        const uint32_t m = has ? (uint32_t)__builtin_ctz(marked) : 0u;
+        marked &= marked - 1u;
+        const uint32_t src = 4u * m + quarter;
+        const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
+        constexpr bool kFast = decltype(mode)::value;
+        // the lane's two halves of 4 samples one after the other (all 8 at
+        // once would hold 24 loaded values and cost a wave per SIMD)
+        const uint32_t ou = bperm_u32(o, src) + 4u * koff;
+        float4 r4 = load4(rs, ou, 0u), b4 = load4(rs, ou, s_b), c4 = load4(rs, ou, s_c);
+        fold_half<kFast, 0>(r4, b4, c4, lws, cb + src, has, best, pos, thr);
+        r4 = load4(rs, ou + 16u, 0u);
+        b4 = load4(rs, ou + 16u, s_b);
+        c4 = load4(rs, ou + 16u, s_c);
+        fold_half<kFast, 1>(r4, b4, c4, lws, cb + src, has, best, pos, thr);
+      }
+    };
+    with_division(fast, [&](auto mode) { phase2(mode); });
+  }
+  // merge the quarters by (ln_a, position): lanes l, l ^ 16, l ^ 32, l ^ 48
+#pragma unroll
+  for (uint32_t d = 16u; d <= 32u; d <<= 1) {
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      const float ob2 = __uint_as_float(bperm_u32(__float_as_uint(best[v]), lane ^ d));
+      const uint32_t op = bperm_u32(pos[v], lane ^ d);
+      const bool take = ob2 < best[v] || (ob2 == best[v] && op < pos[v]);
+      best[v] = take ? ob2 : best[v];
+      pos[v] = take ? op : pos[v];
+    }
+  }
+  if (quarter != 0u || !lane_ok) return true;
+  unsigned long long *slot =
+      reinterpret_cast<unsigned long long *>(M.output) + ((uint64_t)row * samples + k0);
+  unsigned long long seen[8];
+#pragma unroll
+  for (int v = 0; v < 8; v++) seen[v] = slot[v];
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    if (pos[v] != WMH_SENTINEL) {
+      const unsigned long long key = ((unsigned long long)order_key(best[v]) << 32) | pos[v];
+      if (key < seen[v]) atomicMin(slot + v, key);
+    }
+  }
+  return true;
+}
+
 // Matrix whose range of `field` values contains x: the largest m with
 // mats[m].field <= x (mats sorted by field).
 template <uint32_t MatrixDesc::*Field>
@@ -788,7 +1022,7 @@ template <int V>
 __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
-    StripePlan stripes, uint32_t fast_div) {
+    StripePlan stripes, uint32_t fast_div, const uint16_t *__restrict__ bounds) {
   const uint32_t lane = lane_id();
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
@@ -891,8 +1125,29 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
         if constexpr (V == 2) {
           if (piece) {
             const uint64_t col0 = (uint64_t)k * stripes.width;
-            hash_piece_pairs(params, dim, sp, samples, tile, M, a, b, row,
-                             col0 < dim ? (uint32_t)col0 : dim, fast_div != 0u);
+            const uint32_t c0 = col0 < dim ? (uint32_t)col0 : dim;
+            if (bounds != nullptr) {
+              // the row's keyed minima of the lane's 8 samples (quarter
+              // layout); once the earlier pieces left one for every sample
+              // of the tile, the bound can skip most exact evaluations
+              const uint32_t kq = tile * 128u + 8u * (lane & 15u);
+              float sbest[8];
+              bool none = false;
+              const unsigned long long *slot =
+                  reinterpret_cast<const unsigned long long *>(M.output) +
+                  ((uint64_t)row * samples + (kq < samples ? kq : 0u));
+#pragma unroll
+              for (int v = 0; v < 8; v++) {
+                const unsigned long long key = slot[v];
+                none |= kq < samples && key == kNoKey;
+                sbest[v] = key_value(key);
+              }
+              if (__ballot(none) == 0ull &&
+                  hash_piece_bounded(params, bounds, dim, sp, samples, tile, M, a, b, row, c0,
+                                     fast_div != 0u, sbest))
+                continue;
+            }
+            hash_piece_pairs(params, dim, sp, samples, tile, M, a, b, row, c0, fast_div != 0u);
             continue;
           }
         }
@@ -1289,6 +1544,17 @@ hipError_t launch_band_keys(const uint32_t *hashes, uint32_t rows, uint32_t samp
   return hipGetLastError();
 }
 
+hipError_t launch_build_bounds(const float *params, uint32_t dim, uint32_t samples,
+                               uint16_t *bounds, hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  const uint64_t total = (uint64_t)dim * sp;
+  uint64_t blocks = (total + kBlock - 1) / kBlock;
+  if (blocks > 65536u * 4u) blocks = 65536u * 4u;
+  hipLaunchKernelGGL(build_bounds_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
+                     dim3(kBlock), 0, stream, params, dim, samples, sp, bounds);
+  return hipGetLastError();
+}
+
 hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                             unsigned long long *hist, hipStream_t stream) {
   const unsigned grid = std::min<unsigned>(grid_for(total_chunks), 1024u);
@@ -1301,8 +1567,9 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
                              bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
-                             hipEvent_t ev_begin, hipEvent_t ev_end) {
+                             hipEvent_t ev_begin, hipEvent_t ev_end, const uint16_t *bounds) {
   const uint32_t sp = padded_samples(samples);
+  if (samples % 8u != 0u) bounds = nullptr;  // the bounded pieces need 8 whole samples per lane
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
@@ -1332,17 +1599,17 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
     case 1:
       hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div);
+                         (uint32_t)fast_div, bounds);
       break;
     case 2:
       hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div);
+                         (uint32_t)fast_div, bounds);
       break;
     default:
       hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div);
+                         (uint32_t)fast_div, bounds);
       break;
   }
   if (ev_end) (void)hipEventRecord(ev_end, stream);

@@ -133,6 +133,7 @@ struct GraphCache {
   int kernel = 0;
   Tuning tuning{0, 0, 0};
   bool fast_div = false;
+  const uint16_t *bounds = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture = nullptr;
@@ -164,6 +165,13 @@ struct MinhashCudaGenerator_ {
   // every r of the table lies in [2^-60, 2^60], so the kernels may divide
   // with the short correctly rounded sequence (refresh_division_mode)
   bool fast_div = false;
+  // bound table of the skip bound ([dim][sp] fp16, wmh_bound.h; null when
+  // MHCUDA_SKIP_BOUND=0 or it could not be allocated), `bounded` while it
+  // matches the parameters and they pass the bound's range check, and
+  // `skip` the user's switch (mhcuda_set_skip)
+  uint16_t *bounds = nullptr;
+  bool bounded = false;
+  bool skip = true;
   wmh::DevBuf range_flag;
   hipStream_t stream = nullptr;
   // recorded on the legacy default stream by device calls given no stream,
@@ -224,9 +232,16 @@ MHCUDAResult hash_device_batch(MinhashCudaGenerator *gen, std::vector<MatrixDesc
 MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats, int kernel,
                         hipStream_t s);
 
-// Checks the r table on the device after the parameters change and sets
-// gen->fast_div (blocks until the check is done).
+// Checks the tables on the device after the parameters change, sets
+// gen->fast_div and rebuilds the bound table (gen->bounded) when the ranges
+// allow (blocks until done).
 MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen);
+
+// The bound table for a persistent launch, or null when the skip bound is
+// off or unusable for these tables.
+inline const uint16_t *launch_bounds_of(const MinhashCudaGenerator *gen) {
+  return gen->skip && gen->bounded ? gen->bounds : nullptr;
+}
 
 // Device-pointer calls given no stream run on gen->stream, which does not
 // synchronize with the legacy default stream by itself: this makes it wait

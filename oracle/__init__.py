@@ -29,6 +29,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwmh_oracle.so")
 LOGF_CHECK = os.path.join(HERE, "logf_check")
+BOUND_CHECK = os.path.join(HERE, "bound_check")
 SENTINEL = 0xFFFFFFFF
 
 _lock = threading.Lock()
@@ -37,6 +38,7 @@ _lib = None
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
 
 
 def build():
@@ -62,6 +64,10 @@ def lib():
             L.wmh_oracle_calc_omp.argtypes = calc_args + [ctypes.c_int]
             L.wmh_oracle_calc_omp.restype = ctypes.c_int
             L.wmh_oracle_philox.argtypes = [_u32p, _u32p, _u32p]
+            L.wmh_oracle_f16_round_down.argtypes = [_f32p, _u16p, ctypes.c_uint64]
+            L.wmh_oracle_f16_to_f32.argtypes = [_u16p, _f32p, ctypes.c_uint64]
+            L.wmh_oracle_bound_entry.argtypes = [_f32p, _f32p, _u16p, ctypes.c_uint64]
+            L.wmh_oracle_bound_lw.argtypes = [_f32p, _f32p, ctypes.c_uint64]
             L.wmh_oracle_gen_params.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_uint32, _f32p, _f32p, _f32p]
             L.wmh_oracle_gen_params_cols.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
@@ -127,6 +133,38 @@ def calc_f64(rs, ln_cs, betas, weights, cols, indptr):
 def calc_csr(rs, ln_cs, betas, m):
     """calc() on a scipy.sparse CSR matrix."""
     return calc(rs, ln_cs, betas, m.data, m.indices, m.indptr)
+
+
+def f16_round_down(x):
+    """fp16 bit patterns of float32 values rounded toward -inf (the bound
+    table's rounding, wmh_bound.h)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    h = np.empty(x.size, np.uint16)
+    lib().wmh_oracle_f16_round_down(x, h, x.size)
+    return h
+
+
+def f16_to_f32(h):
+    h = np.ascontiguousarray(h, dtype=np.uint16).ravel()
+    x = np.empty(h.size, np.float32)
+    lib().wmh_oracle_f16_to_f32(h, x, h.size)
+    return x
+
+
+def bound_entry(r, lc):
+    """Bound-table entries (fp16 bits) of parameters r, lc (wmh_bound.h)."""
+    r = np.ascontiguousarray(r, dtype=np.float32).ravel()
+    lc = np.ascontiguousarray(lc, dtype=np.float32).ravel()
+    h = np.empty(r.size, np.uint16)
+    lib().wmh_oracle_bound_entry(r, lc, h, r.size)
+    return h
+
+
+def bound_lw(lw):
+    lw = np.ascontiguousarray(lw, dtype=np.float32).ravel()
+    out = np.empty_like(lw)
+    lib().wmh_oracle_bound_lw(lw, out, lw.size)
+    return out
 
 
 def philox(ctr, key):

@@ -39,9 +39,24 @@
 #include <omp.h>
 #endif
 
+#include "../minhashcuda_amd/csrc/wmh_bound.h"
 #include "../minhashcuda_amd/csrc/wmh_math.h"
 
 float wmh_oracle_logf(float x) { return wmh_logf(x); }
+
+// The skip bound's building blocks (wmh_bound.h), vectorised for the tests.
+void wmh_oracle_f16_round_down(const float *x, uint16_t *h, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) h[i] = wmh_f16_round_down(x[i]);
+}
+void wmh_oracle_f16_to_f32(const uint16_t *h, float *x, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) x[i] = wmh_f16_to_f32(h[i]);
+}
+void wmh_oracle_bound_entry(const float *r, const float *lc, uint16_t *h, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) h[i] = wmh_bound_entry(r[i], lc[i]);
+}
+void wmh_oracle_bound_lw(const float *lw, float *out, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) out[i] = wmh_bound_lw(lw[i]);
+}
 
 void wmh_oracle_logf_vec(const float *x, float *y, uint64_t n) {
   for (uint64_t i = 0; i < n; i++) y[i] = wmh_logf(x[i]);

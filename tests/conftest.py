@@ -36,7 +36,7 @@ def _ensure_built():
         from minhashcuda_amd import build
         build.build()
     import oracle
-    if not os.path.exists(oracle.LIB_PATH) or not os.path.exists(oracle.LOGF_CHECK):
+    if not all(os.path.exists(p) for p in (oracle.LIB_PATH, oracle.LOGF_CHECK, oracle.BOUND_CHECK)):
         oracle.build()
 
 

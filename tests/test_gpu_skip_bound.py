@@ -1,0 +1,128 @@
+"""GPU parity of the skip bound (wmh_bound.h): column-stripe pieces whose row
+already holds a minimum for every sample of the tile skip the exact
+evaluation of (nonzero, sample) pairs whose lower bound is above it.  The
+hashes must be the same with the skip on and off and equal the oracle's:
+for sample counts that fill one or two 128-sample tiles or leave part of a
+tile empty, for a count that is not a multiple of 8 (the bound is not used),
+for tables outside the bound's range (not used), and for heavy-tailed,
+zero, infinite and unit weights inside long striped rows."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from minhashcuda_amd import capi as capi_mod
+from minhashcuda_amd import workloads
+
+pytestmark = pytest.mark.gpu
+PERSISTENT = capi_mod.KERNEL_PERSISTENT
+
+
+def skip_state(capi, g):
+    en, act = ctypes.c_int(), ctypes.c_int()
+    assert capi.mhcuda_get_skip(g, ctypes.byref(en), ctypes.byref(act)) == 0
+    return en.value, act.value
+
+
+def long_row_batch(dim, seed, sigma=1.5):
+    rng = np.random.default_rng(seed)
+    mats = []
+    for i in range(3):
+        nnz = np.concatenate([rng.integers(1500, 9000, 10), rng.integers(1, 200, 20)])
+        rng.shuffle(nnz)
+        mats.append(workloads.batched_csr(nnz.astype(np.int64), dim=dim, sigma=sigma,
+                                          seed=seed * 10 + i))
+    return mats
+
+
+@pytest.mark.parametrize("samples", [64, 128, 200, 256, 100])
+def test_skip_on_and_off_equal_the_oracle(libmhcuda, capi, gpu_available, samples):
+    print("samples", samples, flush=True)
+    dim = 1 << 15
+    mats = long_row_batch(dim, samples)
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=samples, devices=1)
+    g = ctypes.c_void_p(gen)
+    try:
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+        assert capi.mhcuda_set_stripes(g, 1) == 0
+        assert capi.mhcuda_set_tuning(g, 2, 1024, 0) == 0
+        assert skip_state(capi, g) == (1, int(samples % 8 == 0))
+        on = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
+        assert capi.mhcuda_set_skip(g, 0) == 0
+        assert skip_state(capi, g) == (0, 0)
+        off = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
+        assert capi.mhcuda_set_skip(g, 1) == 0
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+    for m, a, b in zip(mats, on, off):
+        ref = oracle.calc_omp(*params, m.data, m.indices, m.indptr, 16)[0]
+        assert np.array_equal(a, ref)
+        assert np.array_equal(b, ref)
+
+
+def test_tables_outside_the_bound_range(libmhcuda, capi, gpu_available):
+    """Assigned tables with a beta of 1, an infinite ln_c or a huge ln_c turn
+    the bound off (and the hashes stay exact); valid tables turn it on."""
+    dim, samples = 20000, 128
+    mats = long_row_batch(dim, 3)
+    rs, ln_cs, betas = oracle.datasketch_style_params(dim, samples, seed=4)
+    gen = libmhcuda.minhash_cuda_init(dim, samples, deferred=True, devices=1)
+    g = ctypes.c_void_p(gen)
+    try:
+        assert skip_state(capi, g) == (1, 0)  # nothing assigned yet
+        assert capi.mhcuda_set_stripes(g, 1) == 0
+        assert capi.mhcuda_set_tuning(g, 2, 0, 0) == 0
+        cases = []
+        b2 = betas.copy()
+        b2[3, 11] = 1.0
+        cases.append((rs, ln_cs, b2))
+        c2 = ln_cs.copy()
+        c2[7, 100] = -np.inf
+        cases.append((rs, c2, betas))
+        c3 = ln_cs.copy()
+        c3[0, 5] = np.float32(2.0 ** 61)
+        cases.append((rs, c3, betas))
+        for params in cases:
+            libmhcuda.minhash_cuda_assign_vars(gen, *params)
+            assert skip_state(capi, g) == (1, 0)
+            got = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
+            for m, h in zip(mats, got):
+                assert np.array_equal(h, oracle.calc_omp(*params, m.data, m.indices,
+                                                         m.indptr, 16)[0])
+        libmhcuda.minhash_cuda_assign_vars(gen, rs, ln_cs, betas)
+        assert skip_state(capi, g) == (1, 1)
+        got = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
+        for m, h in zip(mats, got):
+            assert np.array_equal(h, oracle.calc_omp(rs, ln_cs, betas, m.data, m.indices,
+                                                     m.indptr, 16)[0])
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+
+
+def test_extreme_weights_in_striped_rows(libmhcuda, capi, gpu_available):
+    """Heavy-tailed weights (lognormal sigma 6), zeros, subnormals, the
+    largest floats, unit weights (ln w = 0) and an infinite weight early in a
+    long row (its minimum is -inf from then on, so the bound skips everything
+    after it) -- with the skip on, exactly the oracle's hashes."""
+    dim, samples = 1 << 15, 128
+    rng = np.random.default_rng(8)
+    nnz = np.array([12000, 3000, 9000, 5000, 40, 2500, 7000], np.int64)
+    m = workloads.batched_csr(nnz, dim=dim, sigma=6.0, seed=8)
+    special = np.array([0.0, 1e-42, 3e38, 1.0, 1.0, -1.0], np.float32)
+    pick = rng.choice(m.nnz, 3000, replace=False)
+    m.data[pick] = special[rng.integers(0, len(special), 3000)]
+    m.data[m.indptr[0] + 700] = np.inf
+    m.data[m.indptr[3]:m.indptr[4]] = 1.0   # every ln w = 0: t = floor(beta) = 0
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=8, devices=1)
+    g = ctypes.c_void_p(gen)
+    try:
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+        assert capi.mhcuda_set_stripes(g, 1) == 0
+        assert skip_state(capi, g) == (1, 1)
+        for seg in (256, 4096):
+            assert capi.mhcuda_set_tuning(g, 2, seg, 0) == 0
+            got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+            assert np.array_equal(got, oracle.calc(*params, m.data, m.indices, m.indptr)), seg
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)

@@ -314,3 +314,71 @@ def test_touched_columns_oracle_equals_full_table():
     m.data[:] = rng.lognormal(0, 1.5, m.nnz).astype(np.float32)
     m.data[::7] = 0.0  # rows with unselectable entries, some with none at all
     assert np.array_equal(oracle.calc_touched(11, dim, samples, m), oracle.calc_csr(*full, m))
+
+
+# ---------------------------------------------------------------- skip bound
+
+def test_f16_conversions_exhaustive():
+    """fp16 -> f32 equals numpy's (exact) conversion for every pattern, and
+    rounding an fp16 value down returns it unchanged."""
+    h = np.arange(1 << 16, dtype=np.uint32).astype(np.uint16)
+    x = oracle.f16_to_f32(h)
+    ref = h.view(np.float16).astype(np.float32)
+    nan = np.isnan(ref)
+    assert np.array_equal(x[~nan], ref[~nan]) and np.all(np.isnan(x[nan]))
+    fin = ~nan
+    back = oracle.f16_round_down(x[fin])
+    # -0 rounds down to -0 and +0 to +0; every other value is its own pattern
+    assert np.array_equal(back, h[fin])
+
+
+def test_f16_round_down_is_the_largest_fp16_not_above():
+    rng = np.random.default_rng(12)
+    bits = rng.integers(0, 1 << 32, 2_000_000, dtype=np.uint64).astype(np.uint32)
+    x = bits.view(np.float32)
+    x = x[~np.isnan(x)]
+    x = np.concatenate([x, np.float32([0.0, -0.0, 65504, 65505, 65520, -65504, -65505,
+                                       -65520, 1e30, -1e30, 2.0**-24, 2.0**-25, -2.0**-25,
+                                       2.0**-14, -2.0**-14 * 1.0001, np.inf, -np.inf])])
+    h = oracle.f16_round_down(x)
+    v = oracle.f16_to_f32(h).astype(np.float64)
+    xd = x.astype(np.float64)
+    assert np.all(v <= xd)
+    # the next fp16 above (toward +inf) is above x
+    hv = h.view(np.float16)
+    with np.errstate(over="ignore"):
+        up = np.nextafter(hv, np.float16(np.inf)).astype(np.float64)
+    ok = (up > xd) | (np.isposinf(xd) & np.isinf(v))
+    assert np.all(ok | (hv == np.float16(65504)) & (xd > 65504))
+
+
+def test_bound_table_and_lw_terms():
+    """Entries of generated-table parameters sit below lc - r by at least the
+    stated margin; the per-nonzero term handles the non-finite cases."""
+    rs, ln_cs, _ = oracle.gen_params(3, 4000, 64)
+    x = oracle.f16_to_f32(oracle.bound_entry(rs, ln_cs)).astype(np.float64)
+    r, lc = rs.ravel().astype(np.float64), ln_cs.ravel().astype(np.float64)
+    assert np.all(x <= lc - r - 2.0**-17 * (np.abs(lc) + r))
+    assert np.all(x >= lc - r - 2.0**-9 * (np.abs(lc) + r) - 2.0**-14)
+    lw = np.float32([0.0, 1.5, -2.25, np.inf, -np.inf, np.nan, 88.0, -103.0])
+    b = oracle.bound_lw(lw)
+    assert b[0] == 0.0 and b[1] > 1.5 and b[2] > -2.25 and b[3] == np.inf
+    assert b[4] == -np.inf and b[5] == -np.inf
+    assert np.all(b[[1, 2, 6, 7]] >= lw[[1, 2, 6, 7]] + 2.0**-17 * np.abs(lw[[1, 2, 6, 7]]))
+
+
+@pytest.mark.parametrize("variant,expect_violations", [(0, False), (1, True), (2, True),
+                                                       (3, True), (4, True)])
+def test_skip_bound_brute_force(variant, expect_violations):
+    """The skip bound never exceeds the computed ln_a over 2e7 adversarial
+    trials (oracle/bound_check.c); each planted mistake -- the r term dropped,
+    the sign of lw flipped, a nearest-rounded entry, no binary32 margin --
+    is caught."""
+    import subprocess
+    res = subprocess.run([oracle.BOUND_CHECK, str(variant), "20000000"], capture_output=True,
+                         text=True, timeout=300)
+    violations = int(res.stdout.split("violations")[1].split()[0])
+    if expect_violations:
+        assert violations > 0, res.stdout
+    else:
+        assert res.returncode == 0 and violations == 0, res.stdout
