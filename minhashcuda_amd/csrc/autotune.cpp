@@ -200,7 +200,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
       // column stripes (only where the table spans several stripes)
       const Candidate after_wpc = best;
       uint32_t k = 1, min_nnz = 0;
-      for (int mib : {0, 4, 16, 64, 256}) {
+      for (int mib : {0, 4, 8, 16, 32, 64, 256}) {
         if (mib == after_wpc.tuning.stripe_mib) continue;
         wmh::stripe_shape(gen->dim, gen->samples, mib, &k, &min_nnz);
         if (mib != 0 && k < 2) continue;

@@ -811,7 +811,8 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
 //   the quarter's, which comes earlier in CSR order) for every sample and
 //   cannot change the merged result.
 //   Phase 2: each lane evaluates its marked nonzeros exactly (8 samples, the
-//   formula of kernel.cu:103-106) in CSR order, one per round.
+//   formula of kernel.cu:103-106) in CSR order, one per round; lanes with
+//   nothing left to evaluate issue no loads.
 // The four quarters merge by (ln_a, position) and lanes 0-15 merge the result
 // into the row's keyed minima like every piece of a split row.  Requires
 // samples % 8 == 0 (every lane's 8 samples lie in its column's runs).
@@ -966,12 +967,14 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
         // the lane's two halves of 4 samples one after the other (all 8 at
         // once would hold 24 loaded values and cost a wave per SIMD)
         const uint32_t ou = bperm_u32(o, src) + 4u * koff;
-        float4 r4 = load4(rs, ou, 0u), b4 = load4(rs, ou, s_b), c4 = load4(rs, ou, s_c);
-        fold_half<kFast, 0>(r4, b4, c4, lws, cb + src, has, best, pos, thr);
-        r4 = load4(rs, ou + 16u, 0u);
-        b4 = load4(rs, ou + 16u, s_b);
-        c4 = load4(rs, ou + 16u, s_c);
-        fold_half<kFast, 1>(r4, b4, c4, lws, cb + src, has, best, pos, thr);
+        if (has) {
+          float4 r4 = load4(rs, ou, 0u), b4 = load4(rs, ou, s_b), c4 = load4(rs, ou, s_c);
+          fold_half<kFast, 0>(r4, b4, c4, lws, cb + src, true, best, pos, thr);
+          r4 = load4(rs, ou + 16u, 0u);
+          b4 = load4(rs, ou + 16u, s_b);
+          c4 = load4(rs, ou + 16u, s_c);
+          fold_half<kFast, 1>(r4, b4, c4, lws, cb + src, true, best, pos, thr);
+        }
       }
     };
     with_division(fast, [&](auto mode) { phase2(mode); });
