@@ -1,22 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the Weighted MinHash hashing path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched|plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched|plain|large|serve|c1]
 
 A step is one hashing pass over one batch of synthetic CSR input already
 resident in HBM (device entry points of the C ABI, launched on torch's current
-stream).  Prints ONE JSON line: the metric nnz*samples/s, the roofline of the
-dominant kernel (HBM-bound parameter gathers) and the CPU oracle baseline
-timed on this host.
+stream).  Prints ONE JSON line with the metric nnz*samples/s, the roofline of
+the dominant kernel against the resource that binds it (VALU issue or HBM, from
+the rocprofv3 counters committed under profiles/), and the CPU baseline timed
+on this host.
 
-Workloads (128 samples, parameters from the device Philox generator):
+Workloads (parameters from the device Philox generator):
   batched  (default) 64 independent CSR matrices, dim 2^20, rows per matrix
            log-uniform in [1k, 200k], nnz/row Zipf(1.2) clipped to [1, 20000],
-           lognormal(0, 1.5) weights.  `value` is the batched persistent
-           kernel (one launch for all 64 matrices); the plain kernel (one
-           launch per matrix) is timed on the same data beside it.
+           lognormal(0, 1.5) weights, 128 samples.  `value` is the batched
+           persistent kernel (one launch for all 64 matrices, autotuned,
+           hipGraph replay); the plain kernel (one launch per matrix) is timed
+           on the same data beside it.  The same line also carries, under
+           `large`, the 1,000,000 x 12,000,000 share and, under `serve`, the
+           serving loop's p50/p99 latency and throughput (see below).
   plain    100,000 x 2^20 CSR, nnz/row ~ Poisson(100), lognormal(0, 1)
-           weights, plain kernel (one launch, one wavefront per row).
+           weights, 128 samples, plain kernel (one launch, one wavefront per row).
+  large    1,000,000 x 12,000,000 CSR, 16,800 nnz/row, uniform(0, 100]
+           weights, 128 samples, 18.4 GB of parameters resident: one resident
+           65,536-row chunk, plus all 1M rows end to end from pinned host memory.
+  serve    continuous-batching serving loop: Poisson arrivals at a QPS sweep,
+           requests of 1-4096 rows, dim 2^22, 256 samples, 12.9 GB resident.
+  c1       the reference's CPU-runnable case (6400 x 130, test.py recipe).
 """
 import argparse
 import ctypes
@@ -28,6 +38,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "minhashcuda_amd")
 HBM_PEAK_GBS = 8000.0
+# VALU issue ceiling in lane operations per second: the published FP32 vector
+# peak (157.3 TFLOP/s) counts an FMA as two operations and two operations per
+# lane per cycle; one VALU instruction per lane per cycle is a quarter of it
+# (256 CUs x 64 lanes x 2.4 GHz).
+VALU_PEAK_LANE_OPS = 157.3e12 / 4
 METRIC = "nnz*samples/sec (128 samples, bit-exact) batched vs plain; p50/p99 serve latency"
 SAMPLES = 128
 
@@ -36,16 +51,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def parse():
+def parse(argv=None):
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", default="batched", choices=["batched", "plain", "serve", "large", "c1", "copy"],
+    p.add_argument("--workload", default="batched",
+                   choices=["batched", "plain", "serve", "large", "c1", "copy"],
                    help="copy: a plain 4 GiB device-to-device copy (calibrates counters)")
-    p.add_argument("--qps", default="1000,3000,6000,9000,12000",
+    p.add_argument("--qps", default=None,
                    help="serve: comma-separated arrival rates (requests/s)")
-    p.add_argument("--serve-seconds", type=float, default=3.0,
+    p.add_argument("--serve-seconds", type=float, default=None,
                    help="serve: duration of each arrival-rate run")
     p.add_argument("--slots", type=int, default=4,
                    help="serve: launches in flight at once (one stream and worker each)")
@@ -56,25 +72,44 @@ def parse():
     p.add_argument("--wpc", type=int, default=0, help="persistent waves per CU (0: default)")
     p.add_argument("--stripe-mib", type=int, default=-1,
                    help="persistent column-stripe size in MiB (0: off, -1: library default)")
+    p.add_argument("--no-skip", action="store_true",
+                   help="hash without the skip bound (mhcuda_set_skip(0))")
     p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
+    p.add_argument("--no-large", action="store_true",
+                   help="batched: leave the 1M x 12M share out of the line")
+    p.add_argument("--no-serve", action="store_true",
+                   help="batched: leave the serving sweep out of the line")
     p.add_argument("--kernel", default="plain", choices=["plain", "persistent"],
                    help="plain workload: the kernel to time (default: the plain kernel)")
     p.add_argument("--no-graph", action="store_true",
                    help="launch eagerly instead of replaying a captured hipGraph")
     p.add_argument("--no-autotune", action="store_true",
-                   help="batched: keep the default tuning instead of autotuning in warm-up")
+                   help="keep the default tuning instead of autotuning in warm-up")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=10.0,
-                   help="target duration of the single-thread CPU oracle sample")
+    p.add_argument("--cpu-seconds", type=float, default=8.0,
+                   help="target duration of each timed CPU baseline sample")
     p.add_argument("--count", type=int, default=64, help="batched: number of matrices")
-    return p.parse_args()
+    return p.parse_args(argv)
 
+
+# ---------------------------------------------------------------------------
+# host facts and the CPU baseline
 
 def cpu_threads():
+    """Threads of the CPU baseline: the host's CPU share for this process
+    (OMP_NUM_THREADS when the environment sets it, as the GPU hosts do for one
+    GPU's share of their cores), else every CPU this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(affinity())
+
+
+def affinity():
     try:
-        return len(os.sched_getaffinity(0))
+        return sorted(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
-        return os.cpu_count() or 1
+        return list(range(os.cpu_count() or 1))
 
 
 def cpu_model():
@@ -89,65 +124,102 @@ def cpu_model():
     return "unknown"
 
 
+def host_facts():
+    return {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": len(affinity()),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def rows_for(m, units_target, samples):
+    """Smallest row prefix of CSR `m` holding about units_target nnz*samples."""
+    import numpy as np
+    want = units_target / samples
+    n = int(np.searchsorted(m.indptr, want)) if want < m.nnz else m.shape[0]
+    return max(1, min(m.shape[0], n))
+
+
 def cpu_baseline(params, m, samples, target_s, gpu_hashes, what):
-    """Times this repository's C oracle of the hashing path on a bounded row
-    prefix of `m`: single thread, and OpenMP on up to 16 of the cores this
-    process may use; checks the sample's hashes against the GPU's."""
+    """Times the CPU port of the hashing formula (oracle/wmh_cpu_fast.c: the
+    same binary32 operations in the same order, tables transposed so the loop
+    over samples vectorises, rows over OpenMP threads) on bounded row prefixes
+    of `m` -- one thread and the host's CPU share -- plus the scalar C oracle
+    (the reference's [samples][dim] layout) on one thread, and checks every
+    sample's hashes against the GPU's."""
     import numpy as np
     import oracle
-    rs, ln_cs, betas = params
-    n0 = min(m.shape[0], 200)
-    while n0 < m.shape[0] and m[:n0].nnz * samples < 2e7:
-        n0 = min(m.shape[0], n0 * 2)
-    sub = m[:n0]
-    t = time.perf_counter()
-    oracle.calc_csr(rs, ln_cs, betas, sub)
-    rate = max(sub.nnz * samples / max(time.perf_counter() - t, 1e-6), 1.0)
-    want = target_s * rate / samples
-    n1 = int(np.searchsorted(m.indptr, want)) if want < m.nnz else m.shape[0]
-    n1 = max(1, min(m.shape[0], n1))
-    sub = m[:n1]
-    t = time.perf_counter()
-    h1 = oracle.calc_csr(rs, ln_cs, betas, sub)
-    t1 = time.perf_counter() - t
-    threads = min(cpu_threads(), 16)
-    want2 = sub.nnz * threads
-    n2 = int(np.searchsorted(m.indptr, want2)) if want2 < m.nnz else m.shape[0]
-    n2 = max(n1, min(m.shape[0], n2))
-    sub2 = m[:n2]
-    t = time.perf_counter()
-    h2, used = oracle.calc_omp(rs, ln_cs, betas, sub2.data, sub2.indices, sub2.indptr, threads)
-    t2 = time.perf_counter() - t
-    ok = bool(np.array_equal(h2, gpu_hashes[:n2]) and np.array_equal(h1, gpu_hashes[:n1]))
+    params_t = oracle.transpose_params(*params)
+    threads = cpu_threads()
+    out = {}
+
+    def timed(fn, rows):
+        sub = m[:rows]
+        t = time.perf_counter()
+        h = fn(sub)
+        dt = time.perf_counter() - t
+        return sub, h, dt
+
+    # calibrate on a small prefix, then time about target_s of work
+    probe = rows_for(m, 2e8, samples)
+    sub, _, dt = timed(lambda s: oracle.calc_fast(params_t, s.data, s.indices, s.indptr, 1)[0],
+                       probe)
+    rate1 = max(sub.nnz * samples / max(dt, 1e-6), 1.0)
+    n1 = rows_for(m, rate1 * target_s, samples)
+    sub1, h1, t1 = timed(lambda s: oracle.calc_fast(params_t, s.data, s.indices, s.indptr,
+                                                    1)[0], n1)
+    n2 = rows_for(m, rate1 * threads * target_s, samples)
+    used = [0]
+
+    def omp(s):
+        h, used[0] = oracle.calc_fast(params_t, s.data, s.indices, s.indptr, threads)
+        return h
+
+    sub2, h2, t2 = timed(omp, n2)
+    n3 = rows_for(m, rate1 * min(target_s, 3.0) / 20.0, samples)  # the scalar oracle is ~20x slower
+    sub3, h3, t3 = timed(lambda s: oracle.calc_csr(*params, s), n3)
+    ok = bool(np.array_equal(h1, gpu_hashes[:n1]) and np.array_equal(h2, gpu_hashes[:n2]) and
+              np.array_equal(h3, gpu_hashes[:n3]))
     # information only: how often FP64 arithmetic would pick another argmin
-    n3 = max(1, min(n1, 2000))
-    h64 = oracle.calc_f64(rs, ln_cs, betas, m.data, m.indices, m.indptr[:n3 + 1])
-    flips = float(np.mean(np.any(h64 != gpu_hashes[:n3], axis=2)))
-    return {
-        "value": sub2.nnz * samples / t2, "unit": "nnz*samples/s", "cores": int(used),
-        "kind": "port", "cpu_model": cpu_model(),
-        "sample": "%s: first %d rows (%d nnz), OpenMP C oracle, %.1f s" % (what, n2, sub2.nnz, t2),
-        "single_thread_value": sub.nnz * samples / t1,
-        "single_thread_sample": "%s: first %d rows (%d nnz), %.1f s" % (what, n1, sub.nnz, t1),
+    n4 = max(1, min(n3, 2000))
+    h64 = oracle.calc_f64(*params, m.data, m.indices, m.indptr[:n4 + 1])
+    flips = float(np.mean(np.any(h64 != gpu_hashes[:n4], axis=2)))
+    out.update({
+        "value": sub2.nnz * samples / t2, "unit": "nnz*samples/s", "cores": int(used[0]),
+        "kind": "port",
+        "sample": "%s: first %d rows (%d nnz), OpenMP + AVX2 port of the formula on %d threads, "
+                  "%.1f s" % (what, n2, sub2.nnz, used[0], t2),
+        "single_thread_value": sub1.nnz * samples / t1,
+        "single_thread_sample": "%s: first %d rows (%d nnz), the same port on 1 thread, %.1f s"
+                                % (what, n1, sub1.nnz, t1),
+        "scalar_oracle_value": sub3.nnz * samples / t3,
+        "scalar_oracle_sample": "%s: first %d rows (%d nnz), the scalar C oracle (reference "
+                                "[samples][dim] layout), 1 thread, %.1f s"
+                                % (what, n3, sub3.nnz, t3),
         "sample_bit_exact_vs_gpu": ok,
         "fp64_argmin_flip_rate": flips,
         "fp64_flip_sample": "%s: first %d rows, (row, sample) hashes where an FP64 evaluation "
-                            "of the same formula selects another nonzero or t" % (what, n3),
-    }
+                            "of the same formula selects another nonzero or t" % (what, n4),
+        "threads_note": "threads = the host's CPU share for one GPU (OMP_NUM_THREADS when "
+                        "set), not nproc",
+    }, **host_facts())
+    return out
 
+
+# ---------------------------------------------------------------------------
+# the engine through the C ABI
 
 class Engine:
-    def __init__(self, dim, seed, args):
+    def __init__(self, dim, seed, args, samples=SAMPLES):
         import libMHCUDA
         from minhashcuda_amd import capi as capi_mod
         self.L = libMHCUDA
         self.capi_mod = capi_mod
         self.capi = capi_mod.load(os.path.join(PKG, "libMHCUDA.so"))
-        self.gen = libMHCUDA.minhash_cuda_init(dim, SAMPLES, seed=seed, devices=1)
+        self.gen = libMHCUDA.minhash_cuda_init(dim, samples, seed=seed, devices=1)
         self.g = ctypes.c_void_p(self.gen)
         capi_mod.check(self.capi.mhcuda_set_tuning(self.g, args.spl, args.seg, args.wpc), "tuning")
         if getattr(args, "stripe_mib", -1) >= 0:
             capi_mod.check(self.capi.mhcuda_set_stripes(self.g, args.stripe_mib), "stripes")
+        if getattr(args, "no_skip", False):
+            capi_mod.check(self.capi.mhcuda_set_skip(self.g, 0), "skip")
         capi_mod.check(self.capi.mhcuda_set_timing(self.g, 1), "timing")
         capi_mod.check(self.capi.mhcuda_set_graph(self.g, 0 if args.no_graph else 1), "graph")
         self.graph = not args.no_graph
@@ -157,9 +229,11 @@ class Engine:
         self.capi.mhcuda_get_tuning(self.g, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         mib, k, n = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_uint32()
         self.capi.mhcuda_get_stripes(self.g, ctypes.byref(mib), ctypes.byref(k), ctypes.byref(n))
+        en, act = ctypes.c_int(), ctypes.c_int()
+        self.capi.mhcuda_get_skip(self.g, ctypes.byref(en), ctypes.byref(act))
         return {"samples_per_lane": a.value, "segment": b.value, "waves_per_cu": c.value,
                 "stripe_mib": mib.value, "stripes": k.value, "striped_row_min_nnz": n.value,
-                "hip_graph": self.graph}
+                "skip_bound": bool(act.value), "hip_graph": self.graph}
 
     def kernel_ms(self):
         ms = ctypes.c_float()
@@ -174,12 +248,13 @@ class Engine:
         self.L.minhash_cuda_fini(self.gen)
 
 
-def measured_traffic(workload, kernel, alg_bytes, stripes=None):
-    """HBM bytes per launch of `kernel` on this workload as measured by the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
-    profiles/hbm_traffic.json (scaled by a copy-kernel calibration); None when
-    no measurement of this exact configuration (input size, kernel and, for
-    the persistent kernel, number of column stripes) is recorded."""
+# ---------------------------------------------------------------------------
+# rooflines from the committed rocprofv3 counter records
+
+def counter_record(workload, kernel, alg_bytes, stripes=None):
+    """The rocprofv3 record (profiles/hbm_traffic.json) of `kernel` on this
+    workload's input (identified by its algorithmic bytes) at this stripe
+    count, or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
@@ -191,8 +266,56 @@ def measured_traffic(workload, kernel, alg_bytes, stripes=None):
             continue
         if stripes is not None and rec.get("stripes") != stripes:
             continue
-        return rec["traffic_bytes"]
+        return rec
     return None
+
+
+def measured_traffic(workload, kernel, alg_bytes, stripes=None):
+    """HBM bytes per launch of `kernel` on this workload as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
+    profiles/hbm_traffic.json (scaled by a copy-kernel calibration); None when
+    no measurement of this exact configuration (input size, kernel and, for
+    the persistent kernel, number of column stripes) is recorded."""
+    rec = counter_record(workload, kernel, alg_bytes, stripes)
+    return None if rec is None else rec.get("traffic_bytes")
+
+
+def roofline(workload, kname, alg_bytes, avg_ms, stripes):
+    """The dominant kernel against the resource that binds it.  Two ceilings
+    are computed from the counter record of the same kernel, input and stripe
+    count: VALU issue (SQ_INSTS_VALU x 64 lane operations per launch over the
+    launch time, against VALU_PEAK_LANE_OPS) and HBM (FETCH_SIZE + WRITE_SIZE
+    bytes per launch over the launch time, against 8 TB/s).  `bound` is the
+    one closer to its peak; the algorithmic bytes over the launch time are
+    reported beside it as hbm_algorithmic_x (above 1 where the caches serve
+    re-read parameters)."""
+    t = avg_ms * 1e-3
+    rec = counter_record(workload, kname, alg_bytes,
+                         stripes if kname == "wmh_persistent_kernel" else None)
+    traffic = None if rec is None else rec.get("traffic_bytes")
+    insts = None if rec is None else rec.get("valu_insts")
+    hbm_frac = traffic / t / (HBM_PEAK_GBS * 1e9) if traffic else None
+    valu_frac = insts * 64 / t / VALU_PEAK_LANE_OPS if insts else None
+    out = {"kernel": kname, "avg_kernel_ms": avg_ms, "traffic": traffic,
+           "algorithmic_bytes_per_launch": alg_bytes,
+           "hbm_algorithmic_x": alg_bytes / t / (HBM_PEAK_GBS * 1e9),
+           "hbm_traffic_frac": hbm_frac, "valu_frac": valu_frac,
+           "valu_lane_ops_per_launch": insts * 64 if insts else None,
+           "counters": None if rec is None else rec.get("profile")}
+    if valu_frac is not None and (hbm_frac is None or valu_frac >= hbm_frac):
+        out.update({"bound": "valu", "achieved": insts * 64 / t / 1e12,
+                    "peak": VALU_PEAK_LANE_OPS / 1e12, "unit": "T lane-ops/s",
+                    "frac": valu_frac})
+    elif hbm_frac is not None:
+        out.update({"bound": "hbm", "achieved": traffic / t / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": hbm_frac})
+    else:
+        # no counter record of this configuration: the HBM ceiling in
+        # algorithmic bytes, which cache reuse can exceed
+        out.update({"bound": "hbm", "achieved": alg_bytes / t / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s (algorithmic bytes; no counter record of this tuning)",
+                    "frac": alg_bytes / t / (HBM_PEAK_GBS * 1e9)})
+    return out
 
 
 def explicit_tuning(args):
@@ -228,6 +351,9 @@ def timed_steps(torch, run, engine, steps, warmup):
     return time.perf_counter() - t0, kms
 
 
+# ---------------------------------------------------------------------------
+# serving
+
 def serve_requests(n, dim, seed):
     """Serving requests: 1-4096 rows (log-uniform), nnz/row lognormal with
     median 200 and sigma 1 clipped to [1, 50000], lognormal(0, 1) weights."""
@@ -243,9 +369,9 @@ def serve_requests(n, dim, seed):
     return reqs
 
 
-def serve_bench(args, torch):
-    """Continuous-batching serving loop under Poisson arrivals at each rate of
-    --qps: p50/p99 latency (submit -> hashes in host memory) and throughput."""
+def serve_sweep(args, torch, qps_list, seconds):
+    """Continuous-batching serving loop under Poisson arrivals at each rate:
+    p50/p99 latency (submit -> hashes in host memory) and throughput."""
     import threading
     import numpy as np
     import libMHCUDA
@@ -282,8 +408,8 @@ def serve_bench(args, torch):
     server.stats(reset=True)
     results = []
     rng = np.random.default_rng(11)
-    for qps in [float(q) for q in args.qps.split(",")]:
-        n = max(20, int(qps * args.serve_seconds))
+    for qps in qps_list:
+        n = max(20, int(qps * seconds))
         gaps = rng.exponential(1.0 / qps, n)
         tickets = []
         done = threading.Event()
@@ -331,26 +457,50 @@ def serve_bench(args, torch):
             st["mean_requests_per_batch"]))
     server.stop()
     libMHCUDA.minhash_cuda_fini(gen)
+    config = ("serving: Poisson arrivals, requests of 1-4096 rows (log-uniform), dim 2^22, "
+              "nnz/row lognormal(median 200, sigma 1) clipped to 50000, lognormal(0,1) weights, "
+              "256 samples, parameters resident (12.9 GB), %d slots, 4 GiB budget, request "
+              "arrays in %s host memory" % (args.slots, "pinned" if args.pinned else "pageable"))
+    return results, config, len(pool)
+
+
+def serve_summary(results):
+    """The highest offered rate the loop keeps up with (achieved >= 0.98 x
+    offered) and its latencies, plus the saturation throughput of the sweep."""
+    stable = [r for r in results if r["achieved_qps"] >= 0.98 * r["offered_qps"]]
+    top = max(stable, key=lambda r: r["offered_qps"]) if stable else None
+    sat = max(results, key=lambda r: r["nnz_samples_per_sec"])
+    return {
+        "stable_offered_qps": top["offered_qps"] if top else None,
+        "p50_ms": top["p50_ms"] if top else None,
+        "p99_ms": top["p99_ms"] if top else None,
+        "nnz_samples_per_sec_at_stable": top["nnz_samples_per_sec"] if top else None,
+        "saturation_nnz_samples_per_sec": sat["nnz_samples_per_sec"],
+        "saturation_offered_qps": sat["offered_qps"],
+        "saturation_p99_ms": sat["p99_ms"],
+    }
+
+
+def serve_bench(args, torch):
+    qps = [float(q) for q in (args.qps or "1000,3000,6000,9000,12000").split(",")]
+    results, config, pool = serve_sweep(args, torch, qps, args.serve_seconds or 3.0)
     best = max(results, key=lambda r: r["nnz_samples_per_sec"])
     print(json.dumps({
         "metric": METRIC, "workload": "serve", "unit": "nnz*samples/s",
         "value": best["nnz_samples_per_sec"], "higher_is_better": True, "n_gpus": 1,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "serving: Poisson arrivals, requests of 1-4096 rows "
-                               "(log-uniform), dim 2^22, nnz/row lognormal(median 200, sigma 1) "
-                               "clipped to 50000, lognormal(0,1) weights, 256 samples, "
-                               "parameters resident (12.9 GB), %d slots, 4 GiB budget, "
-                               "request arrays in %s host memory"
-                               % (args.slots, "pinned" if args.pinned else "pageable"),
-                   "pool_requests": len(pool)},
-        "sweep": results}), flush=True)
+        "config": {"workload": config, "pool_requests": pool},
+        "summary": serve_summary(results), "sweep": results}), flush=True)
 
+
+# ---------------------------------------------------------------------------
+# the reference's CPU-runnable case
 
 def c1_bench(args, torch):
     """The reference's CPU-runnable case (test.py:93-97, README.md:85-90):
     6400 x 130 CSR from numpy seed 1 (values in [0,100), 80% zeroed), 128
     samples, parameters assigned through assign_random_vars.  Times the whole
-    6400 rows on the CPU oracle (single thread and OpenMP) and on the GPU
+    6400 rows on the CPU (the scalar oracle and the OpenMP port) and on the GPU
     (device-resident input), and checks that the hashes agree."""
     import numpy as np
     import scipy.sparse as sp
@@ -378,9 +528,10 @@ def c1_bench(args, torch):
     t = time.perf_counter()
     h1 = oracle.calc_csr(rs, ln_cs, betas, m)
     t1 = time.perf_counter() - t
-    threads = min(cpu_threads(), 16)
+    threads = cpu_threads()
+    pt = oracle.transpose_params(rs, ln_cs, betas)
     t = time.perf_counter()
-    h2, used = oracle.calc_omp(rs, ln_cs, betas, m.data, m.indices, m.indptr, threads)
+    h2, used = oracle.calc_fast(pt, m.data, m.indices, m.indptr, threads)
     t2 = time.perf_counter() - t
     best = max(res, key=lambda k: res[k]["value"])
     print(json.dumps({
@@ -390,32 +541,36 @@ def c1_bench(args, torch):
         "config": {"workload": "c1: 6400 x 130 CSR (nnz %d), 128 samples, assigned parameters; "
                                "GPU value = best of plain / persistent" % m.nnz},
         "gpu": res,
-        "cpu_baseline": {"value": m.nnz * SAMPLES / t2, "unit": "nnz*samples/s", "cores": used,
-                         "kind": "port", "cpu_model": cpu_model(),
-                         "sample": "all 6400 rows, OpenMP C oracle",
-                         "single_thread_value": m.nnz * SAMPLES / t1,
-                         "single_thread_sample": "all 6400 rows, %.3f s" % t1},
+        "cpu_baseline": dict({"value": m.nnz * SAMPLES / t2, "unit": "nnz*samples/s",
+                              "cores": used, "kind": "port",
+                              "sample": "all 6400 rows, OpenMP + AVX2 port",
+                              "scalar_oracle_value": m.nnz * SAMPLES / t1,
+                              "scalar_oracle_sample": "all 6400 rows, %.3f s" % t1},
+                             **host_facts()),
         "bit_exact": bool(np.array_equal(gpu_h, h1) and np.array_equal(h1, h2)),
     }), flush=True)
     engine.close()
 
 
-def large_bench(args, torch):
+# ---------------------------------------------------------------------------
+# the 1M x 12M share
+
+def large_run(args, torch, steps, warmup, with_cpu):
     """1,000,000 x 12,000,000 CSR, 16,800 nonzeros per row (density 0.0014),
     uniform columns, weights uniform in (0, 100], 128 samples, parameters
     resident (18.4 GB).  Two figures:
       * kernel: the hashing kernel (autotuned on the chunk unless a tuning
         knob is given) on one device-resident 65,536-row chunk (1.1e9
-        nonzeros) -- the roofline figure, reported as `value`;
+        nonzeros) -- the roofline figure;
       * end to end: all 1,000,000 rows through mhcuda_calc_batch from a pinned
         host copy of that chunk (16 matrices of <= 65,536 rows, the last one
         partial), with double-buffered uploads overlapping the hashing --
-        PCIe-inclusive, reported beside it."""
+        PCIe-inclusive, reported beside it.
+    Returns the result dict."""
     import numpy as np
     import scipy.sparse as sp
     from minhashcuda_amd import workloads
     dim, per_row, chunk_rows, total_rows = 12_000_000, 16_800, 65_536, 1_000_000
-    steps = args.steps or 3
     engine = Engine(dim, 1, args)
     cm = engine.capi_mod
     stream = torch.cuda.current_stream()
@@ -431,10 +586,10 @@ def large_bench(args, torch):
     if not args.no_autotune and not explicit_tuning(args):
         kernel, tuned = autotune(engine, desc, stream)
     kname = "wmh_persistent_kernel" if kernel == cm.KERNEL_PERSISTENT else "wmh_plain_kernel"
-    wall, kms = timed_steps(torch, lambda: engine.batch(desc, kernel, stream), engine, steps, 1)
+    wall, kms = timed_steps(torch, lambda: engine.batch(desc, kernel, stream), engine, steps,
+                            warmup)
     avg_ms = sum(kms) / len(kms)
     alg = workloads.algorithmic_bytes(nnz, chunk_rows, SAMPLES)
-    achieved = alg / (avg_ms * 1e-3) / 1e9
     # pinned host copy of the chunk, then the whole 1M rows end to end
     log("large: pinned host copy")
     hw = torch.empty(nnz, dtype=torch.float32, pin_memory=True)
@@ -470,22 +625,16 @@ def large_bench(args, torch):
     for n in lengths:
         same = same and np.array_equal(host_h[row0:row0 + n], dev_h[:n])
         row0 += n
+    tuning = engine.tuning()
     result = {
-        "metric": METRIC, "workload": "large", "unit": "nnz*samples/s",
-        "value": nnz * SAMPLES * steps / wall, "steps": steps, "warmup": 1,
-        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "n_gpus": 1,
-        "scaling": "weak", "dtype": "f32", "data": "synthetic", "vs_baseline": None,
+        "value": nnz * SAMPLES * steps / wall, "unit": "nnz*samples/s", "steps": steps,
+        "warmup": warmup, "ms_per_step": wall * 1e3 / steps,
         "config": dict({"workload": "large: 1,000,000 x 12,000,000 CSR, 16,800 nnz/row, "
                                     "uniform(0,100] weights, 128 samples, parameters resident "
                                     "(18.4 GB); value = the autotuned hashing kernel on one "
                                     "resident 65,536-row chunk (1.1e9 nnz)",
-                        "chunk_rows": chunk_rows, "chunk_nnz": nnz}, **engine.tuning()),
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("large", kname, alg,
-                                                 engine.tuning()["stripes"]
-                                                 if kname == "wmh_persistent_kernel" else None),
-                     "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms},
+                        "chunk_rows": chunk_rows, "chunk_nnz": nnz}, **tuning),
+        "roofline": roofline("large", kname, alg, avg_ms, tuning["stripes"]),
         "end_to_end": {"rows": total_rows, "nnz": e2e_nnz,
                        "seconds": min(ends), "value": e2e_units / min(ends),
                        "unit": "nnz*samples/s (PCIe-inclusive, pinned host input)",
@@ -493,7 +642,7 @@ def large_bench(args, torch):
     }
     if tuned:
         result["autotune"] = tuned
-    if not args.no_cpu_baseline:
+    if with_cpu:
         log("large: cpu baseline (retrieving 18.4 GB of parameters)")
         params = engine.L.minhash_cuda_retrieve_vars(engine.gen)
         cut = 600
@@ -502,11 +651,24 @@ def large_bench(args, torch):
         result["cpu_baseline"] = cpu_baseline(params, m0, SAMPLES, args.cpu_seconds,
                                               dev_h[:cut], "first rows of the chunk")
     engine.close()
-    print(json.dumps(result), flush=True)
+    del w, c, r, out, hw, hc, hr, hout
+    torch.cuda.empty_cache()
+    return result
 
 
-def main():
-    args = parse()
+def large_bench(args, torch):
+    res = large_run(args, torch, args.steps or 3, args.warmup if args.warmup is not None else 1,
+                    not args.no_cpu_baseline)
+    print(json.dumps(dict({"metric": METRIC, "workload": "large", "n_gpus": 1,
+                           "higher_is_better": True, "scaling": "weak", "dtype": "f32",
+                           "data": "synthetic", "vs_baseline": None}, **res)), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# the uneven batch (default) and the plain configuration
+
+def main(argv=None):
+    args = parse(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine hashes on a single MI355X "
               "(multi-GPU row sharding is not built); "
@@ -597,34 +759,33 @@ def main():
     units = nnz * SAMPLES
     avg_ms = sum(kms) / len(kms)
     alg = workloads.algorithmic_bytes(nnz, rows, SAMPLES)
-    achieved = alg / (avg_ms * 1e-3) / 1e9
+    tuning = engine.tuning()
+    roof = roofline(args.workload, kname, alg, avg_ms, tuning["stripes"])
+    roof["timed_span"] = ("hipGraph replay: descriptor upload, plan, init, %s, fix-up (HIP "
+                          "events around the graph launch)" % kname
+                          if engine.graph and kernel == cm.KERNEL_PERSISTENT else
+                          "HIP events around the %s launch(es)" % kname)
     result.update({
         "value": units * steps / wall, "steps": steps, "warmup": warmup,
         "ms_per_step": wall * 1e3 / steps,
         "config": dict({"workload": desc, "rows": rows, "nnz": nnz, "samples": SAMPLES,
-                        "matrices": len(tensors), "parallelism": "single GPU"},
-                       **engine.tuning()),
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, kname, alg,
-                                                 engine.tuning()["stripes"]
-                                                 if kname == "wmh_persistent_kernel" else None),
-                     "algorithmic_bytes_per_launch": alg, "avg_kernel_ms": avg_ms,
-                     "timed_span": ("hipGraph replay: descriptor upload, plan, init, %s, "
-                                    "fix-up (HIP events around the graph launch)" % kname
-                                    if engine.graph and kernel == cm.KERNEL_PERSISTENT else
-                                    "HIP events around the %s launch(es)" % kname)},
+                        "matrices": len(tensors), "parallelism": "single GPU"}, **tuning),
+        "roofline": roof,
     })
     if tuned is not None:
         result["autotune"] = tuned
     if args.workload == "batched" and not args.no_plain:
+        # the plain kernel is timed on fewer steps (it is about 5x slower and
+        # its launches vary little)
+        psteps = min(steps, 5)
         ref_out = [o.clone() for o in outs]
         for o in outs:
             o.fill_(0)
         pwall, pkms = timed_steps(torch, lambda: engine.batch(descs, cm.KERNEL_PLAIN, stream),
-                                  engine, steps, warmup)
+                                  engine, psteps, 1)
         same = all(torch.equal(a, b) for a, b in zip(ref_out, outs))
-        result["plain"] = {"value": units * steps / pwall, "ms_per_step": pwall * 1e3 / steps,
+        result["plain"] = {"value": units * psteps / pwall, "steps": psteps,
+                           "ms_per_step": pwall * 1e3 / psteps,
                            "avg_kernel_ms": sum(pkms) / len(pkms),
                            "launches_per_step": len(tensors),
                            "identical_to_batched": bool(same)}
@@ -637,7 +798,7 @@ def main():
             # the CPU sample: the first rows of the first matrix, copied back
             w, c, r = tensors[0]
             rp = r.cpu().numpy().view(np.uint32).astype(np.int64)
-            cut = int(np.searchsorted(rp, 60_000_000))
+            cut = int(np.searchsorted(rp, 120_000_000))
             cut = max(1, min(lengths[0], cut))
             import scipy.sparse as sp
             m0 = sp.csr_matrix((w[:rp[cut]].cpu().numpy(),
@@ -651,6 +812,17 @@ def main():
             result["cpu_baseline"] = cpu_baseline(params, mats[0], SAMPLES, args.cpu_seconds,
                                                   gh, "plain workload")
     engine.close()
+    del tensors, outs, descs
+    torch.cuda.empty_cache()
+    if args.workload == "batched" and not args.no_large:
+        log("large share")
+        result["large"] = large_run(args, torch, 3, 1, False)
+    if args.workload == "batched" and not args.no_serve:
+        log("serving sweep")
+        qps = [float(q) for q in (args.qps or "2000,4000,6000,8000,10000,12000").split(",")]
+        sweep, config, pool = serve_sweep(args, torch, qps, args.serve_seconds or 2.0)
+        result["serve"] = dict(serve_summary(sweep), config=config, pool_requests=pool,
+                               sweep=sweep)
     print(json.dumps(result), flush=True)
 
 

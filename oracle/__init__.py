@@ -30,10 +30,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwmh_oracle.so")
 LOGF_CHECK = os.path.join(HERE, "logf_check")
 BOUND_CHECK = os.path.join(HERE, "bound_check")
+FAST_LIB_PATH = os.path.join(HERE, "libwmh_cpu_fast.so")
 SENTINEL = 0xFFFFFFFF
 
 _lock = threading.Lock()
 _lib = None
+_fast = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -128,6 +130,40 @@ def calc_omp(rs, ln_cs, betas, weights, cols, indptr, threads):
 def calc_f64(rs, ln_cs, betas, weights, cols, indptr):
     """FP64 twin of calc() (information only: how often FP32 flips the argmin)."""
     return _run(lib().wmh_oracle_calc_f64, rs, ln_cs, betas, weights, cols, indptr)[0]
+
+
+def transpose_params(rs, ln_cs, betas):
+    """The three [samples][dim] tables as contiguous [dim][samples] copies
+    (the layout of the timed CPU baseline, calc_fast)."""
+    return tuple(np.ascontiguousarray(np.asarray(t, np.float32).T) for t in (rs, ln_cs, betas))
+
+
+def calc_fast(params_t, weights, cols, indptr, threads):
+    """The timed CPU baseline (wmh_cpu_fast.c): the same hashes as calc(),
+    from transposed tables (transpose_params), vectorised over samples and
+    spread over `threads` OpenMP threads.  Returns (hashes, threads used)."""
+    global _fast
+    with _lock:
+        if _fast is None:
+            if not os.path.exists(FAST_LIB_PATH):
+                build()
+            F = ctypes.CDLL(FAST_LIB_PATH)
+            F.wmh_cpu_calc_fast.restype = ctypes.c_int
+            F.wmh_cpu_calc_fast.argtypes = [_f32p, _f32p, _f32p, ctypes.c_uint32,
+                                            ctypes.c_uint32, _f32p, _u32p, _u64p,
+                                            ctypes.c_uint64, _u32p, ctypes.c_int]
+            _fast = F
+    rs_t, lc_t, b_t = params_t
+    dim, samples = rs_t.shape
+    w, c, r = _csr_arrays(weights, cols, indptr)
+    length = len(r) - 1
+    out = np.empty((length, samples, 2), dtype=np.uint32)
+    if w.size == 0:
+        w = np.zeros(1, np.float32)
+        c = np.zeros(1, np.uint32)
+    used = _fast.wmh_cpu_calc_fast(rs_t, lc_t, b_t, dim, samples, w, c, r, length,
+                                   out.reshape(-1), threads)
+    return out, used
 
 
 def calc_csr(rs, ln_cs, betas, m):

@@ -57,6 +57,45 @@ def test_cpu_model_is_a_name():
     assert isinstance(name, str) and name
 
 
+def test_roofline_picks_the_binding_resource(tmp_path, monkeypatch):
+    """bound = the ceiling closer to its peak among VALU issue (SQ_INSTS_VALU x
+    64 lanes against 157.3e12 / 4 lane operations per second) and HBM (fetched +
+    written bytes against 8 TB/s); without a record, the algorithmic bytes."""
+    recs = {"workloads": {"w": [
+        {"kernel": "wmh_persistent_kernel", "algorithmic_bytes": 1e12, "stripes": 48, "traffic_bytes": 4e9,
+         "valu_insts": 3e9, "profile": "p.md"},
+        {"kernel": "wmh_persistent_kernel", "algorithmic_bytes": 1e12, "stripes": 7, "traffic_bytes": 7e9,
+         "valu_insts": 1e8}]}}
+    (tmp_path / "profiles").mkdir()
+    (tmp_path / "profiles" / "hbm_traffic.json").write_text(json.dumps(recs))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    r = bench.roofline("w", "wmh_persistent_kernel", 1e12, 1.0, 48)  # 1 ms
+    valu = 3e9 * 64 / 1e-3 / bench.VALU_PEAK_LANE_OPS
+    hbm = 4e9 / 1e-3 / 8e12
+    assert r["bound"] == "valu" and r["frac"] == pytest.approx(valu)
+    assert r["valu_frac"] == pytest.approx(valu) and r["hbm_traffic_frac"] == pytest.approx(hbm)
+    assert r["achieved"] == pytest.approx(3e9 * 64 / 1e-3 / 1e12)
+    assert r["hbm_algorithmic_x"] == pytest.approx(1e12 / 1e-3 / 8e12)
+    r = bench.roofline("w", "wmh_persistent_kernel", 1e12, 1.0, 7)
+    assert r["bound"] == "hbm" and r["frac"] == pytest.approx(7e9 / 1e-3 / 8e12)
+    assert r["achieved"] == pytest.approx(7e9 / 1e-3 / 1e9) and r["unit"] == "GB/s"
+    r = bench.roofline("w", "wmh_persistent_kernel", 1e12, 1.0, 9)
+    assert r["bound"] == "hbm" and r["traffic"] is None
+    assert r["frac"] == pytest.approx(1e12 / 1e-3 / 8e12)
+
+
+def test_serve_summary_picks_the_highest_stable_rate():
+    sweep = [{"offered_qps": 1000.0, "achieved_qps": 1001.0, "p50_ms": 1, "p99_ms": 2,
+              "nnz_samples_per_sec": 1e10},
+             {"offered_qps": 4000.0, "achieved_qps": 3950.0, "p50_ms": 3, "p99_ms": 9,
+              "nnz_samples_per_sec": 4e10},
+             {"offered_qps": 8000.0, "achieved_qps": 6000.0, "p50_ms": 90, "p99_ms": 300,
+              "nnz_samples_per_sec": 6e10}]
+    s = bench.serve_summary(sweep)
+    assert s["stable_offered_qps"] == 4000.0 and (s["p50_ms"], s["p99_ms"]) == (3, 9)
+    assert s["saturation_nnz_samples_per_sec"] == 6e10 and s["saturation_p99_ms"] == 300
+
+
 @pytest.mark.parametrize("name", ["batched", "plain", "large"])
 def test_committed_lines_keep_the_contract(name):
     line = load_line(name)
@@ -65,14 +104,30 @@ def test_committed_lines_keep_the_contract(name):
     assert line["steps"] > 0 and line["value"] > 0 and "workload" in line["config"]
     roof = line["roofline"]
     assert ROOFLINE_KEYS <= set(roof)
-    assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
-    # achieved = algorithmic bytes of one launch / its average duration
-    gbs = roof["algorithmic_bytes_per_launch"] / (roof["avg_kernel_ms"] * 1e-3) / 1e9
-    assert roof["achieved"] == pytest.approx(gbs, rel=1e-9)
-    assert roof["frac"] == pytest.approx(roof["achieved"] / roof["peak"], rel=1e-12)
+    assert roof["bound"] in ("valu", "hbm") and 0 < roof["frac"] <= 1
+    # the line's roofline is what bench.roofline computes from the committed
+    # counter record of the same kernel, input and stripe count
+    stripes = line["config"].get("stripes")
+    again = bench.roofline(name, roof["kernel"], roof["algorithmic_bytes_per_launch"],
+                           roof["avg_kernel_ms"], stripes)
+    for key in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
+        assert again[key] == pytest.approx(roof[key]) if isinstance(roof[key], float) \
+            else again[key] == roof[key], key
     cpu = line["cpu_baseline"]
     assert CPU_KEYS <= set(cpu) and cpu["kind"] in ("port", "reference")
     assert cpu["cores"] >= 1 and cpu["unit"] == line["unit"]
+    assert cpu["sample_bit_exact_vs_gpu"] is True
+
+
+def test_default_line_carries_the_large_share_and_serving():
+    line = load_line("batched")
+    assert line["batched_over_plain"] >= 3.0 and line["plain"]["identical_to_batched"]
+    large = line["large"]
+    assert large["value"] > 0 and 0 < large["roofline"]["frac"] <= 1
+    assert large["end_to_end"]["identical_to_resident_chunk"] is True
+    serve = line["serve"]
+    assert serve["stable_offered_qps"] and serve["p50_ms"] <= serve["p99_ms"]
+    assert serve["saturation_nnz_samples_per_sec"] > 0
 
 
 @pytest.mark.parametrize("name", ["batched", "plain", "large"])

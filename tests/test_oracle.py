@@ -23,6 +23,7 @@ import scipy.sparse as sp
 from scipy.stats import gamma, uniform
 
 import oracle
+from minhashcuda_amd import workloads
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 S = oracle.SENTINEL
@@ -382,3 +383,21 @@ def test_skip_bound_brute_force(variant, expect_violations):
         assert violations > 0, res.stdout
     else:
         assert res.returncode == 0 and violations == 0, res.stdout
+
+
+def test_fast_cpu_baseline_equals_oracle():
+    """The timed CPU baseline (transposed tables, vectorised, OpenMP) gives
+    the oracle's hashes bit for bit, edge weights included."""
+    dim, samples = 3000, 128
+    params = oracle.gen_params(7, dim, samples)
+    nnz = np.random.default_rng(2).integers(0, 400, 300).astype(np.int64)
+    m = workloads.batched_csr(nnz, dim=dim, sigma=2.0, seed=3)
+    m.data[::13] = 0.0
+    m.data[5] = np.inf
+    m.data[7] = -1.0
+    m.data[9] = np.float32(1e-42)
+    ref = oracle.calc_csr(*params, m)
+    for threads in (1, 4):
+        got, used = oracle.calc_fast(oracle.transpose_params(*params), m.data, m.indices,
+                                     m.indptr, threads)
+        assert used == threads and np.array_equal(got, ref)
