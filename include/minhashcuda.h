@@ -289,6 +289,20 @@ MHCUDAResult mhcuda_jaccard_pairs(const MinhashCudaGenerator *gen, const uint32_
 MHCUDAResult mhcuda_lsh_band_keys(const MinhashCudaGenerator *gen, const uint32_t *hashes,
                                   uint32_t rows, uint32_t bands, uint64_t *keys, void *stream);
 
+/* LSH candidate pairs from band keys (device memory, [rows][bands] as written
+ * by mhcuda_lsh_band_keys): the distinct row pairs (i < j) whose keys agree in
+ * at least one band, skipping the reserved key and buckets of more than
+ * max_bucket rows (max_bucket >= 2), as pair codes i << 32 | j sorted
+ * ascending.  Writes the first min(*count, capacity) codes to `pairs` (device
+ * memory; may be NULL when capacity is 0) and the number of distinct pairs to
+ * *count (host memory): call again with a larger buffer when *count exceeds
+ * capacity.  Returns after completion; `stream` orders the work after the
+ * caller's (NULL: after the legacy default stream). */
+MHCUDAResult mhcuda_lsh_candidates(const MinhashCudaGenerator *gen, const uint64_t *keys,
+                                   uint32_t rows, uint32_t bands, uint32_t max_bucket,
+                                   uint64_t *pairs, uint64_t capacity, uint64_t *count,
+                                   void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Serving loop: continuous batching of CSR requests over one generator.     */
 

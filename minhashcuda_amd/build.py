@@ -36,6 +36,7 @@ HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-pthread"
 
 SOURCES = [
     ("wmh_kernels.hip", True),
+    ("lsh.hip", True),
     ("generator.cpp", False),
     ("serve.cpp", False),
     ("autotune.cpp", False),

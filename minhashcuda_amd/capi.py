@@ -25,7 +25,7 @@ EXPORTS = (
     "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_stripes", "mhcuda_get_stripes",
     "mhcuda_set_skip", "mhcuda_get_skip",
     "mhcuda_autotune", "mhcuda_jaccard_pairs",
-    "mhcuda_lsh_band_keys", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
+    "mhcuda_lsh_band_keys", "mhcuda_lsh_candidates", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
     "mhcuda_params_file_info", "mhcuda_set_graph",
     "mhcuda_graph_captures",
     "mhcuda_set_timing", "mhcuda_kernel_time", "mhcuda_inject_fault",
@@ -121,6 +121,9 @@ def load(path):
     L.mhcuda_jaccard_pairs.argtypes = [vp, vp, vp, vp, u32, vp, vp]
     L.mhcuda_lsh_band_keys.restype = i32
     L.mhcuda_lsh_band_keys.argtypes = [vp, vp, u32, u32, vp, vp]
+    L.mhcuda_lsh_candidates.restype = i32
+    L.mhcuda_lsh_candidates.argtypes = [vp, vp, u32, u32, u32, vp, ctypes.c_uint64,
+                                        ctypes.POINTER(ctypes.c_uint64), vp]
     L.mhcuda_save_random_vars.restype = i32
     L.mhcuda_save_random_vars.argtypes = [vp, ctypes.c_char_p, i32]
     L.mhcuda_load_random_vars.restype = i32

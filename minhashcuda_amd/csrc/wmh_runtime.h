@@ -147,6 +147,12 @@ struct GraphCache {
   void release();
 };
 
+// Grow-only device buffers of LSH candidate generation (lsh.hip).
+struct LshBuffers {
+  DevBuf keys_in, rows_in, keys_sorted, rows_sorted, counts, offsets, pairs, pairs_sorted, temp,
+      scalars;
+};
+
 }  // namespace wmh
 
 struct MinhashCudaGenerator_ {
@@ -195,6 +201,7 @@ struct MinhashCudaGenerator_ {
   wmh::GraphCache graph;  // hipGraph replay of device-pointer calls (opt-in)
   std::map<uint64_t, wmh::TunedChoice> tuned;  // autotuner cache by batch shape
   wmh::DevBuf tune_buf;
+  wmh::LshBuffers lsh;  // mhcuda_lsh_candidates
   int kernel = mhcudaKernelAuto;
   // optional timing of the hashing kernel(s) of the last call
   bool timing = false;

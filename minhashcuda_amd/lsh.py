@@ -44,3 +44,35 @@ def candidate_pairs(keys, max_bucket=64):
     pairs = np.concatenate(found).astype(np.int64)
     code = np.unique(pairs[:, 0] * rows + pairs[:, 1])
     return np.stack([code // rows, code % rows], axis=1)
+
+
+def candidate_pairs_gpu(gen, keys, max_bucket=64, lib=None):
+    """candidate_pairs on the GPU (mhcuda_lsh_candidates): `keys` is a torch
+    tensor on the device, [rows][bands] int64 holding the uint64 band keys
+    of mhcuda_lsh_band_keys.  Returns the same int64 array [n][2] (i < j,
+    sorted) as candidate_pairs."""
+    import ctypes
+    import os
+
+    import torch
+
+    from . import capi as capi_mod
+    lib = lib or capi_mod.load(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                            "libMHCUDA.so"))
+    rows, bands = keys.shape
+    keys = keys.contiguous()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    count = ctypes.c_uint64(0)
+    capacity = max(1024, 4 * rows * bands)
+    while True:
+        out = torch.empty(capacity, dtype=torch.int64, device=keys.device)
+        capi_mod.check(lib.mhcuda_lsh_candidates(ctypes.c_void_p(gen), keys.data_ptr(), rows,
+                                                 bands, max_bucket, out.data_ptr(), capacity,
+                                                 ctypes.byref(count), stream),
+                       "mhcuda_lsh_candidates")
+        if count.value <= capacity:
+            break
+        capacity = count.value
+    codes = out[:count.value].cpu().numpy().view(np.uint64)
+    return np.stack([(codes >> np.uint64(32)).astype(np.int64),
+                     (codes & np.uint64(0xFFFFFFFF)).astype(np.int64)], axis=1)

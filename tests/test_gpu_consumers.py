@@ -66,7 +66,8 @@ def test_candidates_are_enriched_in_similar_pairs(capi, hashed):
     keys = torch.empty((h.shape[0], 16), dtype=torch.int64, device="cuda")
     assert capi.mhcuda_lsh_band_keys(ctypes.c_void_p(gen), dh.data_ptr(), h.shape[0], 16,
                                      keys.data_ptr(), None) == 0
-    pairs = lsh.candidate_pairs(keys)
+    pairs = lsh.candidate_pairs_gpu(gen, keys)
+    assert np.array_equal(pairs, lsh.candidate_pairs(keys))
     assert (8, 9) in {tuple(p) for p in pairs.tolist()}
     rng = np.random.default_rng(1)
     cand = pairs[rng.choice(len(pairs), min(len(pairs), 2000), replace=False)]
@@ -74,3 +75,45 @@ def test_candidates_are_enriched_in_similar_pairs(capi, hashed):
     jc = np.mean([oracle.weighted_jaccard(data[a], data[b]) for a, b in cand])
     jr = np.mean([oracle.weighted_jaccard(data[a], data[b]) for a, b in rand if a != b])
     assert jc > jr + 0.15, (jc, jr)
+
+
+@pytest.mark.parametrize("bands,max_bucket", [(32, 64), (16, 3), (128, 2)])
+def test_gpu_candidates_equal_numpy_on_hashes(capi, hashed, bands, max_bucket):
+    """GPU candidate generation (mhcuda_lsh_candidates) equals the numpy
+    restatement lsh.candidate_pairs on the 6400 x 130 hashes, including the
+    empty row's reserved keys and buckets above max_bucket."""
+    from minhashcuda_amd import lsh
+    gen, data, h = hashed
+    dh = torch.from_numpy(h.view(np.int32)).cuda()
+    keys = torch.empty((h.shape[0], bands), dtype=torch.int64, device="cuda")
+    assert capi.mhcuda_lsh_band_keys(ctypes.c_void_p(gen), dh.data_ptr(), h.shape[0], bands,
+                                     keys.data_ptr(), None) == 0
+    got = lsh.candidate_pairs_gpu(gen, keys, max_bucket=max_bucket)
+    want = lsh.candidate_pairs(keys, max_bucket=max_bucket)
+    assert len(want) > 0 and np.array_equal(got, want)
+
+
+def test_gpu_candidates_equal_numpy_on_a_million_rows(capi, hashed):
+    """1.2M rows x 8 bands of keys with planted collisions (buckets of about
+    3 rows), reserved keys and buckets larger than max_bucket: the GPU pairs
+    equal the numpy ones; a too small output buffer reports the count."""
+    from minhashcuda_amd import lsh
+    gen = hashed[0]
+    rows, bands = 1_200_000, 8
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, rows // 3, (rows, bands)).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    k[rng.random((rows, bands)) < 0.01] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    k[:70, 3] = np.uint64(12345)          # one bucket of 70 rows (> 64: skipped)
+    k[100:164, 5] = np.uint64(777)        # one bucket of exactly 64 rows (kept)
+    keys = torch.from_numpy(k.view(np.int64)).cuda()
+    got = lsh.candidate_pairs_gpu(gen, keys, max_bucket=64)
+    want = lsh.candidate_pairs(k, max_bucket=64)
+    assert len(want) > 1_000_000 and np.array_equal(got, want)
+    count = ctypes.c_uint64(0)
+    out = torch.empty(10, dtype=torch.int64, device="cuda")
+    assert capi.mhcuda_lsh_candidates(ctypes.c_void_p(gen), keys.data_ptr(), rows, bands, 64,
+                                      out.data_ptr(), 10, ctypes.byref(count), None) == 0
+    assert count.value == len(want)
+    codes = out.cpu().numpy().view(np.uint64)
+    assert np.array_equal(codes, (want[:10, 0].astype(np.uint64) << np.uint64(32)) |
+                          want[:10, 1].astype(np.uint64))
