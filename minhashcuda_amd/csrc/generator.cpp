@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cinttypes>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -24,6 +25,7 @@
 #include <vector>
 
 #include "minhashcuda.h"
+#include "wmh_bound.h"
 #include "wmh_kernels.h"
 #include "wmh_runtime.h"
 
@@ -269,7 +271,7 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen) {
   int verbosity = gen->verbosity;
   gen->fast_div = false;
   gen->bounded = false;
-  HIPCH(gen->range_flag.reserve(sizeof(uint32_t)), mhcudaMemoryAllocationFailure);
+  HIPCH(gen->range_flag.reserve(3 * sizeof(uint32_t)), mhcudaMemoryAllocationFailure);
   HIPCH(hipMemsetAsync(gen->range_flag.ptr, 0, sizeof(uint32_t), gen->stream), mhcudaRuntimeError);
   HIPCH(launch_params_range(gen->params, gen->dim, gen->samples, (uint32_t *)gen->range_flag.ptr,
                             gen->stream),
@@ -280,10 +282,33 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen) {
   HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
   gen->fast_div = (bad & kRangeDiv) == 0;
   if (gen->bounds && bad == 0) {
-    HIPCH(launch_build_bounds(gen->params, gen->dim, gen->samples, gen->bounds, gen->stream),
+    // the code range: smallest and largest entry over the table
+    uint32_t *minmax = (uint32_t *)gen->range_flag.ptr + 1;
+    const uint32_t init[2] = {0xffffffffu, 0u};
+    HIPCH(hipMemcpyAsync(minmax, init, sizeof(init), hipMemcpyHostToDevice, gen->stream),
+          mhcudaMemoryCopyError);
+    HIPCH(launch_bound_range(gen->params, gen->dim, gen->samples, minmax, gen->stream),
           mhcudaRuntimeError);
+    uint32_t got[2] = {0u, 0u};
+    HIPCH(hipMemcpyAsync(got, minmax, sizeof(got), hipMemcpyDeviceToHost, gen->stream),
+          mhcudaMemoryCopyError);
     HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
-    gen->bounded = true;
+    const float mn = wmh_order_value(got[0]), mx = wmh_order_value(got[1]);
+    // an entry of -inf (|ln_c - r| beyond the fp16 range) leaves no finite
+    // lower end for the codes: hash without the bound
+    if (got[0] <= got[1] && std::isfinite(mn) && std::isfinite(mx)) {
+      float lo = 0.0f, delta = 0.0f;
+      wmh_code_range(mn, mx, &lo, &delta);
+      HIPCH(launch_build_codes(gen->params, gen->dim, gen->samples, lo, delta, gen->bounds,
+                               gen->stream),
+            mhcudaRuntimeError);
+      HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
+      gen->code_scale = 1.0f / delta;
+      gen->code_t0 = 2049.0f - lo / delta;
+      gen->bounded = true;
+      if (verbosity > 1)
+        printf("bound codes: entries in [%g, %g], lo %g, delta %g\n", mn, mx, lo, delta);
+    }
   }
   return mhcudaSuccess;
 }
@@ -305,7 +330,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
                     gc.tuning.waves_per_cu == gen->tuning.waves_per_cu &&
                     gc.tuning.stripe_mib == gen->tuning.stripe_mib &&
                     gc.fast_div == gen->fast_div &&
-                    gc.bounds == launch_bounds_of(gen) && gc.key.size() == mats.size() &&
+                    same_table(gc.bounds, launch_bounds_of(gen)) && gc.key.size() == mats.size() &&
                     std::memcmp(gc.key.data(), mats.data(), sizeof(MatrixDesc) * mats.size()) == 0;
   if (!same) {
     if (!gc.capture)
@@ -430,11 +455,11 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
     INFO("failed to allocate %zu bytes for the parameters\n", param_bytes);
     FAIL(mhcudaMemoryAllocationFailure);
   }
-  // the bound table of the skip bound (a sixth of the parameter bytes); the
+  // the bound table of the skip bound (a twelfth of the parameter bytes); the
   // engine hashes without it when it cannot be allocated or is switched off
   const char *skip_env = std::getenv("MHCUDA_SKIP_BOUND");
   if (!(skip_env && std::strcmp(skip_env, "0") == 0)) {
-    if (hipMalloc((void **)&gen->bounds, (size_t)dim * gen->sp * sizeof(uint16_t)) != hipSuccess) {
+    if (hipMalloc((void **)&gen->bounds, (size_t)dim * gen->sp) != hipSuccess) {
       (void)hipGetLastError();
       gen->bounds = nullptr;
       INFO("no memory for the bound table: hashing without the skip bound\n");
@@ -903,7 +928,7 @@ MHCUDAResult mhcuda_set_skip(MinhashCudaGenerator *gen, int enable) {
 MHCUDAResult mhcuda_get_skip(const MinhashCudaGenerator *gen, int *enabled, int *active) {
   if (!gen) return mhcudaInvalidArguments;
   if (enabled) *enabled = gen->skip ? 1 : 0;
-  if (active) *active = wmh::launch_bounds_of(gen) != nullptr && gen->samples % 8 == 0;
+  if (active) *active = wmh::launch_bounds_of(gen).codes != nullptr && gen->samples % 8 == 0;
   return mhcudaSuccess;
 }
 

@@ -91,3 +91,87 @@ WMH_HD float wmh_bound_lw(float lw) {
 // Range of the parameter tables within which the bound holds (checked on the
 // device whenever the tables change; generated tables always pass).
 #define WMH_BOUND_MAX_ABS_LC 0x1p60f
+
+// ---------------------------------------------------------------------------
+// Byte codes of the bound table.
+//
+// The engine keeps entry X (the fp16 value above) of every (column, sample)
+// as one byte c = min(255, floor((X - lo) / delta)), with delta = 2^e >= 1/16
+// and lo a multiple of delta at or below every X of the table, so that
+// lo + c delta <= X.  For a sample whose running minimum is thr and a nonzero
+// whose term is lwb, (nonzero, sample) is skipped when c >= T + cB, where
+//   T  = floor((thr - lo) / delta) + 1     (so lo + T delta > thr),
+//   cB = ceil(lwb / delta)                 (so cB delta >= lwb):
+// then X >= lo + (T + cB) delta > thr + lwb, and the computed ln_a, above
+// X - lwb, is above thr -- strictly, so ties keep the earlier position.
+// Clamps, each toward evaluating more: T to [-2048, 2303] and cB to
+// [-2048, 2304] (lwb = -inf: -2048, never selectable; lwb = +inf: 2304,
+// never skipped).  A finite lwb is below 104 in magnitude (the logarithms of
+// positive finite floats), so |cB| <= 1664 never reaches the clamps, and
+// T = 2303 skips nothing such a nonzero brings (T + cB >= 639 > 255).  With
+// T' = T + 2048 in [0, 4351] the 16-bit sum
+//   H = c + (0x4800 - T') + (0x4000 - cB)
+// lies in [0x6E01, 0x90FF], and its bit 15 is set exactly when c >= T + cB:
+// the kernel tests two samples per 32-bit add, one per 16-bit half.
+#define WMH_CODE_T_MAX 4351u
+
+// Byte code of an entry (lo, delta = 1 / inv_delta as above).  The rounded
+// subtraction may land on the next integer; the exact check lo + c delta <= X
+// (a multiple of delta below 2^21 delta: exact) takes it back.
+WMH_HD uint32_t wmh_bound_code(uint16_t x16, float lo, float delta, float inv_delta) {
+  WMH_NO_CONTRACT
+  const float x = wmh_f16_to_f32(x16);
+  float q = __builtin_floorf((x - lo) * inv_delta);
+  q = q > 0.0f ? q : 0.0f;
+  q = q < 255.0f ? q : 255.0f;
+  uint32_t c = (uint32_t)q;
+  if (c > 0u && lo + (float)c * delta > x) c--;
+  return c;
+}
+
+// T' = T + 2048 of a running minimum thr, with scale = 1 / delta and
+// t0 = 2049 - lo / delta (both exact): the floor of the correctly rounded
+// fma is never below the exact T' (rounding does not cross the integer
+// below), then clamped to [0, 4351] (NaN: 0).
+WMH_HD uint32_t wmh_code_threshold(float thr, float scale, float t0) {
+  float q = __builtin_fmaf(thr, scale, t0);
+  q = q > 0.0f ? q : 0.0f;
+  q = q < 4351.0f ? q : 4351.0f;
+  return (uint32_t)__builtin_floorf(q);
+}
+
+// 0x4000 - cB of a nonzero's term lwb (scale = 1 / delta <= 16: lwb * scale
+// is an exact power-of-two scaling).
+WMH_HD uint32_t wmh_code_lw(float lwb, float scale) {
+  WMH_NO_CONTRACT
+  float q = __builtin_ceilf(lwb * scale);
+  q = q > -2048.0f ? q : -2048.0f;  // -inf (NaN is never produced): -2048
+  q = q < 2304.0f ? q : 2304.0f;
+  return (uint32_t)(0x4000 - (int32_t)q);
+}
+
+// 1 when (code c, threshold T', nonzero term) is skipped.
+WMH_HD uint32_t wmh_code_skips(uint32_t c, uint32_t tprime, uint32_t lw_term) {
+  return ((c + (0x4800u - tprime) + lw_term) >> 15) & 1u;
+}
+
+// Order-preserving unsigned key of a non-NaN float, and back.
+WMH_HD uint32_t wmh_order_bits(float x) {
+  const uint32_t u = wmh_f2u(x);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+WMH_HD float wmh_order_value(uint32_t key) {
+  return wmh_u2f((key & 0x80000000u) ? (key & 0x7fffffffu) : ~key);
+}
+
+// delta and lo of a table whose entries lie in [mn, mx] (finite): delta the
+// smallest power of two from 1/16 up whose 255 steps cover 90% of the range
+// (entries above lo + 255 delta take code 255), lo = mn rounded down to a
+// multiple of delta; |lo| / delta <= 2^20 (|mn| <= 65504).
+WMH_HD void wmh_code_range(float mn, float mx, float *lo, float *delta) {
+  WMH_NO_CONTRACT
+  float d = 0.0625f;
+  while (255.0f * d < 0.9f * (mx - mn) && d < 65536.0f) d *= 2.0f;
+  *delta = d;
+  *lo = __builtin_floorf(mn / d) * d;
+}

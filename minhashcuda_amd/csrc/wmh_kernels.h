@@ -131,10 +131,26 @@ constexpr uint32_t kRangeDiv = 1u, kRangeBound = 2u;
 hipError_t launch_params_range(const float *params, uint32_t dim, uint32_t samples,
                                uint32_t *bad, hipStream_t stream);
 
-// Fills the bound table [dim][sp] (fp16 bits, wmh_bound.h) from the
-// parameter table; valid only while both range checks pass.
-hipError_t launch_build_bounds(const float *params, uint32_t dim, uint32_t samples,
-                               uint16_t *bounds, hipStream_t stream);
+// The byte-coded bound table of the skip bound (wmh_bound.h): `codes`
+// [dim][sp] bytes, and the scale 1 / delta and offset t0 = 2049 - lo / delta
+// of its thresholds.  codes == null: the skip bound is not used.
+struct CodeTable {
+  const uint8_t *codes;
+  float scale;
+  float t0;
+};
+
+// Smallest and largest bound entry (wmh_bound_entry) over the real samples
+// of the table, as order-preserving keys (wmh_order_bits) into minmax[0]
+// (atomic minimum) and minmax[1] (atomic maximum); the caller sets them to
+// 0xffffffff and 0 first.  Valid only while both range checks pass.
+hipError_t launch_bound_range(const float *params, uint32_t dim, uint32_t samples,
+                              uint32_t *minmax, hipStream_t stream);
+
+// Fills codes [dim][sp] with wmh_bound_code of every entry for the given lo
+// and delta (padded samples: 255).
+hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t samples, float lo,
+                              float delta, uint8_t *codes, hipStream_t stream);
 
 // Scatter one [samples][ncols] FP32 table (rows of pitch ncols) into
 // params[(j0 + jj) * 3 * sp + which * sp + k].
@@ -164,15 +180,15 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
 // `striped` (device memory, one entry per row of the batch) receives the
 // striped rows when tuning.stripe_mib asks for stripes; it may be null
 // otherwise.
-// `bounds`: the bound table when the skip bound may be used (both range
-// checks passed, samples a multiple of 8), else null; only the
-// 2-samples-per-lane kernel's stripe pieces use it.
+// `table`: the byte-coded bound table when the skip bound may be used (both
+// range checks passed), else codes = null; it is used only when samples is
+// a multiple of 8, and only by the 2-samples-per-lane kernel's stripe pieces.
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
                              bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
                              hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
-                             const uint16_t *bounds = nullptr);
+                             CodeTable table = CodeTable{nullptr, 0.0f, 0.0f});
 
 // Weighted-Jaccard estimates of row pairs (pairs[2i], pairs[2i+1] index the
 // rows of hash tables a and b, each [rows][samples][2]).

@@ -192,20 +192,51 @@ __global__ __launch_bounds__(kBlock) void params_range_kernel(const float *__res
   }
 }
 
-// The bound table: entry (j, k) = wmh_bound_entry(r, ln_c) of column j,
-// sample k, laid out [dim][sp] (one 2 * sp-byte run per column); padded
-// samples get +inf (every bound test skips them).
-__global__ __launch_bounds__(kBlock) void build_bounds_kernel(const float *__restrict__ params,
-                                                              uint32_t dim, uint32_t samples,
-                                                              uint32_t sp,
-                                                              uint16_t *__restrict__ bounds) {
+// Smallest and largest bound entry of the real samples (order-preserving
+// keys of their float values), one atomic pair per wavefront.
+__global__ __launch_bounds__(kBlock) void bound_range_kernel(const float *__restrict__ params,
+                                                             uint32_t dim, uint32_t samples,
+                                                             uint32_t sp,
+                                                             uint32_t *__restrict__ minmax) {
+  const uint64_t total = (uint64_t)dim * samples;
+  uint32_t lo = 0xffffffffu, hi = 0u;
+  for (uint64_t idx = (uint64_t)blockIdx.x * kBlock + threadIdx.x; idx < total;
+       idx += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t j = (uint32_t)(idx / samples);
+    const uint32_t k = (uint32_t)(idx - (uint64_t)j * samples);
+    const float *col = params + (uint64_t)j * 3u * sp;
+    const uint32_t key = wmh_order_bits(wmh_f16_to_f32(wmh_bound_entry(col[k], col[2u * sp + k])));
+    lo = min(lo, key);
+    hi = max(hi, key);
+  }
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, d));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, d));
+  }
+  if (lane_id() == 0u) {
+    atomicMin(minmax, lo);
+    atomicMax(minmax + 1, hi);
+  }
+}
+
+// The byte-coded bound table: entry (j, k) = wmh_bound_code of column j,
+// sample k, laid out [dim][sp] (one sp-byte run per column); padded samples
+// get 255.
+__global__ __launch_bounds__(kBlock) void build_codes_kernel(const float *__restrict__ params,
+                                                             uint32_t dim, uint32_t samples,
+                                                             uint32_t sp, float lo, float delta,
+                                                             uint8_t *__restrict__ codes) {
   const uint64_t total = (uint64_t)dim * sp;
+  const float inv_delta = 1.0f / delta;  // a power of two: exact
   for (uint64_t idx = (uint64_t)blockIdx.x * kBlock + threadIdx.x; idx < total;
        idx += (uint64_t)gridDim.x * kBlock) {
     const uint32_t j = (uint32_t)(idx / sp);
     const uint32_t k = (uint32_t)(idx - (uint64_t)j * sp);
     const float *col = params + (uint64_t)j * 3u * sp;
-    bounds[idx] = k < samples ? wmh_bound_entry(col[k], col[2u * sp + k]) : (uint16_t)0x7c00u;
+    codes[idx] = (uint8_t)(k < samples ? wmh_bound_code(wmh_bound_entry(col[k], col[2u * sp + k]),
+                                                        lo, delta, inv_delta)
+                                       : 255u);
   }
 }
 
@@ -801,24 +832,23 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
 // pieces whose row already holds a keyed minimum for every sample of the
 // tile (the earlier stripes' pieces wrote them).  Quarter-wave layout: lane
 // l hashes samples k0 = 128 tile + 8 (l % 16) .. k0 + 7 at the positions
-// cb + 4m + l / 16 (m = 0..15) of each 64-position chunk, so one 16-byte load
-// per lane covers four nonzeros x 128 samples of the 2-byte bound table.
-//   Phase 1: for every (nonzero, sample) the bound L = X - lwb is compared
-//   with the sample's threshold, min(the row's keyed minimum read at the
-//   start, the quarter's own minimum): a lane marks nonzero m when any of its
-//   8 samples has L <= threshold (NaN: marked).  A computed ln_a is never
-//   below L, so an unmarked nonzero is above the row's minimum (or not below
-//   the quarter's, which comes earlier in CSR order) for every sample and
-//   cannot change the merged result.
+// cb + 4m + l / 16 (m = 0..15) of each 64-position chunk, so one 8-byte load
+// per lane covers four nonzeros x 128 samples of the byte-coded bound table.
+//   Phase 1: for every (nonzero, sample) the code c is tested against the
+//   sample's threshold T' -- from the smaller of the row's keyed minimum read
+//   at the start and the quarter's own minimum -- and the nonzero's term
+//   (wmh_code_skips, two samples per 32-bit add): a lane marks nonzero m
+//   when any of its 8 samples is not skipped.  A skipped (nonzero, sample)
+//   has a computed ln_a above the threshold, so an unmarked nonzero is above
+//   the row's minimum (or above the quarter's, which comes earlier in CSR
+//   order) for every sample and cannot change the merged result.
 //   Phase 2: each lane evaluates its marked nonzeros exactly (8 samples, the
 //   formula of kernel.cu:103-106) in CSR order, one per round; lanes with
-//   nothing left to evaluate issue no loads.
+//   nothing left to evaluate issue no loads.  The thresholds then follow the
+//   quarter's new minima.
 // The four quarters merge by (ln_a, position) and lanes 0-15 merge the result
 // into the row's keyed minima like every piece of a split row.  Requires
 // samples % 8 == 0 (every lane's 8 samples lie in its column's runs).
-// phase-1 steps whose bound loads are issued together (8: +3.7% over 4 on the
-// uneven batch; 16 needs 157 VGPRs and loses a wave per SIMD, -12%)
-constexpr int kBoundGroup = 8;
 
 // The value of a keyed minimum (order_key(ln_a) << 32 | position).
 __device__ __forceinline__ float key_value(unsigned long long key) {
@@ -826,11 +856,23 @@ __device__ __forceinline__ float key_value(unsigned long long key) {
   return __uint_as_float((h & 0x80000000u) ? (h & 0x7fffffffu) : ~h);
 }
 
-__device__ __forceinline__ float half_lo(uint32_t x) {
-  return (float)__builtin_bit_cast(_Float16, (uint16_t)(x & 0xffffu));
+// Phase-1 steps whose code loads are issued together (all 16 of a chunk
+// need 147 VGPRs and lose a wave per SIMD).
+constexpr int kBoundGroup = 8;
+
+// Two byte codes of a 32-bit word (bytes 2i, 2i + 1) as 16-bit halves.
+template <int I>
+__device__ __forceinline__ uint32_t code_pair(uint32_t w) {
+  return __builtin_amdgcn_perm(0u, w, I == 0 ? 0x0c010c00u : 0x0c030c02u);
 }
-__device__ __forceinline__ float half_hi(uint32_t x) {
-  return (float)__builtin_bit_cast(_Float16, (uint16_t)(x >> 16));
+
+// Packed 0x4800 - T' of the lane's 8 samples, two per word (wmh_bound.h).
+__device__ __forceinline__ void code_thresholds(const float (&thr)[8], float scale, float t0,
+                                                uint32_t (&tpair)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    tpair[i] = (0x4800u - wmh_code_threshold(thr[2 * i], scale, t0)) |
+               ((0x4800u - wmh_code_threshold(thr[2 * i + 1], scale, t0)) << 16);
 }
 
 // Folds samples H*4 .. H*4 + 3 of the lane's 8 (one 16-byte load of each
@@ -857,7 +899,8 @@ __device__ __forceinline__ void fold_half(const float4 &r4, const float4 &b4, co
 }
 
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
-                                                   const uint16_t *__restrict__ bounds,
+                                                   const uint8_t *__restrict__ codes,
+                                                   float code_scale, float code_t0,
                                                    uint32_t dim, uint32_t sp, uint32_t samples,
                                                    uint32_t tile, const MatrixDesc &M, uint32_t a,
                                                    uint32_t b, uint32_t row, uint32_t col0_in,
@@ -868,16 +911,13 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
   const uint32_t k0 = tile * 128u + 8u * (lane & 15u);
   const bool lane_ok = k0 < samples;
   const uint32_t koff = lane_ok ? k0 : 0u;
-  const uint32_t block_bytes = 12u * sp, bound_bytes = 2u * sp;
+  const uint32_t block_bytes = 12u * sp;  // the code table holds sp bytes per column
   const uint32_t s_b = uniform(sp * 4u), s_c = uniform(sp * 8u);  // beta, ln_c arrays
-  // 1.0f from a value the compiler cannot fold (sp < 2^31), so that
-  // fma(X, one, -lwb) keeps its fp16 operand: one v_fma_mix_f32 per bound
-  const float one = __uint_as_float(0x3f800000u | (sp >> 31));
   // column offsets c - col0 up to `span` keep every lane's bytes below 2^32
   const uint32_t span = (0xffffffffu - 12u * sp) / block_bytes;
   // the resources must live in SGPRs: their bases are made wave-uniform
   const uint64_t pb = reinterpret_cast<uint64_t>(params) + (uint64_t)col0 * block_bytes;
-  const uint64_t bb = reinterpret_cast<uint64_t>(bounds) + (uint64_t)col0 * bound_bytes;
+  const uint64_t bb = reinterpret_cast<uint64_t>(codes) + (uint64_t)col0 * sp;
   char *pbase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(pb >> 32)) << 32) |
                                          uniform((uint32_t)pb));
   char *bbase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(bb >> 32)) << 32) |
@@ -888,22 +928,6 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       __builtin_amdgcn_make_buffer_rsrc(bbase, (short)0, (int)0xffffffff, kRawBufferWord3);
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
-  // every gather goes through the two resources with 32-bit offsets; a piece
-  // holding a column too far from the stripe for that (an unsorted row in a
-  // table of 4 GiB or more) is left to hash_piece_pairs
-  // (the offsets are those of the gathers below: lanes without a column
-  // below dim gather from the stripe's first column, or from column 0 when
-  // the stripe starts at dim, which does not fit and leaves the piece to
-  // hash_piece_pairs)
-  for (uint32_t cb = a; cb < b; cb += kWave) {
-    const uint32_t n = uniform(min((uint32_t)kWave, b - cb));
-    uint32_t c = col0 < dim ? col0 : 0u;
-    if (lane < n) {
-      const uint32_t cc = cols[cb + lane];
-      if (cc < dim) c = cc;
-    }
-    if (__ballot(c - col0 > span) != 0ull) return false;
-  }
   float best[8], thr[8];
   uint32_t pos[8];
 #pragma unroll
@@ -912,59 +936,73 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     pos[v] = WMH_SENTINEL;
     thr[v] = slot_best[v];
   }
+  uint32_t tpair[4];
+  code_thresholds(thr, code_scale, code_t0, tpair);
+  // the next chunk's columns and weights are loaded while this one is tested
+  uint32_t next_c = dim;
+  float next_w = 0.0f;
+  if (lane < b - a) {
+    next_c = cols[a + lane];
+    next_w = weights[a + lane];
+  }
   for (uint32_t cb = a; cb < b; cb += kWave) {
     const uint32_t n = uniform(min((uint32_t)kWave, b - cb));
+    const uint32_t cc = next_c;
+    const float wv = next_w;
+    next_c = dim;
+    if (cb + kWave < b && lane < b - cb - kWave) {
+      next_c = cols[cb + kWave + lane];
+      next_w = weights[cb + kWave + lane];
+    }
     // lanes without a selectable nonzero gather from a valid block (the
-    // stripe's first, or column 0 for a stripe past dim) and are skipped by
-    // every bound test (lwb = -inf) and never selected (lw = NaN)
+    // stripe's first, or column 0 for a stripe past dim) and are never
+    // selected (lw = NaN)
     uint32_t c = col0 < dim ? col0 : 0u;
     float lw = __builtin_nanf("");
     float lwb = -__builtin_inff();
-    if (lane < n) {
-      const uint32_t cc = cols[cb + lane];
-      if (cc < dim) {
-        c = cc;
-        lw = wmh_logf(weights[cb + lane]);  // once per nonzero (kernel.cu:95)
-        lwb = wmh_bound_lw(lw);
-      }
+    if (lane < n && cc < dim) {
+      c = cc;
+      lw = wmh_logf(wv);  // once per nonzero (kernel.cu:95)
+      lwb = wmh_bound_lw(lw);
     }
+    // every gather goes through the two resources with 32-bit offsets; a
+    // piece holding a column too far from the stripe for that (an unsorted
+    // row in a table of 4 GiB or more, or a stripe starting at dim, whose
+    // lanes without a column gather from column 0) is left to
+    // hash_piece_pairs, before anything of it is written
+    if (__ballot(c - col0 > span) != 0ull) return false;
     const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
     const uint32_t rel = c - col0;
-    const uint32_t o = rel * block_bytes, ob = rel * bound_bytes;
-    const uint32_t steps = uniform((n + 3u) >> 2);
+    const uint32_t o = rel * block_bytes, ob = rel * sp;
+    const uint32_t lw_term = wmh_code_lw(lwb, code_scale);
+    const uint32_t lw_pair = lw_term | (lw_term << 16);
     // phase 1: the lane's marked nonzeros, bit m for position cb + 4m + quarter
+    // (positions at or past n, and lanes past the samples, are masked off)
     uint32_t marked = 0u;
-    {
-      for (uint32_t m0 = 0; m0 < steps; m0 += kBoundGroup) {
-        uint4 x[kBoundGroup];
-        float lb[kBoundGroup];
+    const uint32_t steps = uniform((n + 3u) >> 2);
+    for (uint32_t m0 = 0; m0 < steps; m0 += kBoundGroup) {
+      uint2 x[kBoundGroup];
 #pragma unroll
-        for (int u = 0; u < kBoundGroup; u++) {
-          const uint32_t src = 4u * (m0 + u) + quarter;  // lanes past n (or 63) read valid data
-          lb[u] = __uint_as_float(bperm_u32(__float_as_uint(lwb), src));
-          x[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
-                                               rb, bperm_u32(ob, src) + 2u * koff, 0, 0));
-        }
+      for (int u = 0; u < kBoundGroup; u++) {
+        const uint32_t src = 4u * (m0 + u) + quarter;  // lanes past n read valid data
+        x[u] = __builtin_bit_cast(
+            uint2, __builtin_amdgcn_raw_buffer_load_b64(rb, bperm_u32(ob, src) + koff, 0, 0));
+      }
 #pragma unroll
-        for (int u = 0; u < kBoundGroup; u++) {
-          const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
-          // the lanes whose sample v of this nonzero is not skipped, one
-          // ballot per sample; X - lwb as fma(X, 1, -lwb) (exact product:
-          // the same rounding as the subtraction, read straight from the
-          // fp16 half)
-          unsigned long long any = 0ull;
-#pragma unroll
-          for (int h = 0; h < 4; h++) {
-            any |= __ballot(!(__builtin_fmaf(half_lo(w[h]), one, -lb[u]) > thr[2 * h]));
-            any |= __ballot(!(__builtin_fmaf(half_hi(w[h]), one, -lb[u]) > thr[2 * h + 1]));
-          }
-          const uint32_t m = m0 + u;
-          const bool hit = (any >> lane) & 1ull;
-          if (hit && m < steps && 4u * m + quarter < n && lane_ok) marked |= 1u << m;
-        }
+      for (int u = 0; u < kBoundGroup; u++) {
+        const uint32_t lp = bperm_u32(lw_pair, 4u * (m0 + u) + quarter);
+        // bit 15 of each half: that sample skipped (wmh_code_skips)
+        const uint32_t h = (code_pair<0>(x[u].x) + tpair[0] + lp) &
+                           (code_pair<1>(x[u].x) + tpair[1] + lp) &
+                           (code_pair<0>(x[u].y) + tpair[2] + lp) &
+                           (code_pair<1>(x[u].y) + tpair[3] + lp);
+        if ((h & 0x80008000u) != 0x80008000u) marked |= 1u << (m0 + u);
       }
     }
-    // phase 2: exact evaluation of the marked nonzeros, lowest position first
+    const uint32_t live = n > quarter ? (n - quarter + 3u) >> 2 : 0u;  // positions of this quarter
+    marked &= lane_ok ? (1u << live) - 1u : 0u;
+    const bool any_marked = __ballot(marked != 0u) != 0ull;
+  // phase 2: exact evaluation of the marked nonzeros, lowest position first
     auto phase2 = [&](auto mode) {
       for (;;) {
         const bool has = marked != 0u;
@@ -987,7 +1025,10 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
         }
       }
     };
-    with_division(fast, [&](auto mode) { phase2(mode); });
+    if (any_marked) {
+      with_division(fast, [&](auto mode) { phase2(mode); });
+      code_thresholds(thr, code_scale, code_t0, tpair);
+    }
   }
   // merge the quarters by (ln_a, position): lanes l, l ^ 16, l ^ 32, l ^ 48
 #pragma unroll
@@ -1035,7 +1076,8 @@ template <int V>
 __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
-    StripePlan stripes, uint32_t fast_div, const uint16_t *__restrict__ bounds) {
+    StripePlan stripes, uint32_t fast_div, const uint8_t *__restrict__ codes, float code_scale,
+    float code_t0) {
   const uint32_t lane = lane_id();
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
@@ -1139,7 +1181,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
           if (piece) {
             const uint64_t col0 = (uint64_t)k * stripes.width;
             const uint32_t c0 = col0 < dim ? (uint32_t)col0 : dim;
-            if (bounds != nullptr) {
+            if (codes != nullptr) {
               // the row's keyed minima of the lane's 8 samples (quarter
               // layout); once the earlier pieces left one for every sample
               // of the tile, the bound can skip most exact evaluations
@@ -1156,8 +1198,8 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
                 sbest[v] = key_value(key);
               }
               if (__ballot(none) == 0ull &&
-                  hash_piece_bounded(params, bounds, dim, sp, samples, tile, M, a, b, row, c0,
-                                     fast_div != 0u, sbest))
+                  hash_piece_bounded(params, codes, code_scale, code_t0, dim, sp, samples,
+                                     tile, M, a, b, row, c0, fast_div != 0u, sbest))
                 continue;
             }
             hash_piece_pairs(params, dim, sp, samples, tile, M, a, b, row, c0, fast_div != 0u);
@@ -1557,14 +1599,25 @@ hipError_t launch_band_keys(const uint32_t *hashes, uint32_t rows, uint32_t samp
   return hipGetLastError();
 }
 
-hipError_t launch_build_bounds(const float *params, uint32_t dim, uint32_t samples,
-                               uint16_t *bounds, hipStream_t stream) {
+hipError_t launch_bound_range(const float *params, uint32_t dim, uint32_t samples,
+                              uint32_t *minmax, hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  const uint64_t total = (uint64_t)dim * samples;
+  uint64_t blocks = (total + kBlock - 1) / kBlock;
+  if (blocks > 65536u) blocks = 65536u;
+  hipLaunchKernelGGL(bound_range_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
+                     dim3(kBlock), 0, stream, params, dim, samples, sp, minmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t samples, float lo,
+                              float delta, uint8_t *codes, hipStream_t stream) {
   const uint32_t sp = padded_samples(samples);
   const uint64_t total = (uint64_t)dim * sp;
   uint64_t blocks = (total + kBlock - 1) / kBlock;
   if (blocks > 65536u * 4u) blocks = 65536u * 4u;
-  hipLaunchKernelGGL(build_bounds_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
-                     dim3(kBlock), 0, stream, params, dim, samples, sp, bounds);
+  hipLaunchKernelGGL(build_codes_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
+                     dim3(kBlock), 0, stream, params, dim, samples, sp, lo, delta, codes);
   return hipGetLastError();
 }
 
@@ -1580,9 +1633,10 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
                              bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
-                             hipEvent_t ev_begin, hipEvent_t ev_end, const uint16_t *bounds) {
+                             hipEvent_t ev_begin, hipEvent_t ev_end, CodeTable table) {
   const uint32_t sp = padded_samples(samples);
-  if (samples % 8u != 0u) bounds = nullptr;  // the bounded pieces need 8 whole samples per lane
+  // the bounded pieces need 8 whole samples per lane
+  const uint8_t *codes = samples % 8u == 0u ? table.codes : nullptr;
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
@@ -1612,17 +1666,17 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
     case 1:
       hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, bounds);
+                         (uint32_t)fast_div, codes, table.scale, table.t0);
       break;
     case 2:
       hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, bounds);
+                         (uint32_t)fast_div, codes, table.scale, table.t0);
       break;
     default:
       hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, bounds);
+                         (uint32_t)fast_div, codes, table.scale, table.t0);
       break;
   }
   if (ev_end) (void)hipEventRecord(ev_end, stream);

@@ -133,7 +133,7 @@ struct GraphCache {
   int kernel = 0;
   Tuning tuning{0, 0, 0};
   bool fast_div = false;
-  const uint16_t *bounds = nullptr;
+  CodeTable bounds{nullptr, 0.0f, 0.0f};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture = nullptr;
@@ -171,11 +171,13 @@ struct MinhashCudaGenerator_ {
   // every r of the table lies in [2^-60, 2^60], so the kernels may divide
   // with the short correctly rounded sequence (refresh_division_mode)
   bool fast_div = false;
-  // bound table of the skip bound ([dim][sp] fp16, wmh_bound.h; null when
-  // MHCUDA_SKIP_BOUND=0 or it could not be allocated), `bounded` while it
-  // matches the parameters and they pass the bound's range check, and
-  // `skip` the user's switch (mhcuda_set_skip)
-  uint16_t *bounds = nullptr;
+  // byte-coded bound table of the skip bound ([dim][sp] bytes, wmh_bound.h;
+  // null when MHCUDA_SKIP_BOUND=0 or it could not be allocated) with the
+  // scale and offset of its thresholds, `bounded` while it matches the
+  // parameters and they pass the bound's range check, and `skip` the user's
+  // switch (mhcuda_set_skip)
+  uint8_t *bounds = nullptr;
+  float code_scale = 0.0f, code_t0 = 0.0f;
   bool bounded = false;
   bool skip = true;
   wmh::DevBuf range_flag;
@@ -246,8 +248,13 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen);
 
 // The bound table for a persistent launch, or null when the skip bound is
 // off or unusable for these tables.
-inline const uint16_t *launch_bounds_of(const MinhashCudaGenerator *gen) {
-  return gen->skip && gen->bounded ? gen->bounds : nullptr;
+inline CodeTable launch_bounds_of(const MinhashCudaGenerator *gen) {
+  if (!(gen->skip && gen->bounded)) return CodeTable{nullptr, 0.0f, 0.0f};
+  return CodeTable{gen->bounds, gen->code_scale, gen->code_t0};
+}
+
+inline bool same_table(const CodeTable &x, const CodeTable &y) {
+  return x.codes == y.codes && x.scale == y.scale && x.t0 == y.t0;
 }
 
 // Device-pointer calls given no stream run on gen->stream, which does not

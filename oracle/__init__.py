@@ -41,6 +41,7 @@ _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
 def build():
@@ -70,6 +71,14 @@ def lib():
             L.wmh_oracle_f16_to_f32.argtypes = [_u16p, _f32p, ctypes.c_uint64]
             L.wmh_oracle_bound_entry.argtypes = [_f32p, _f32p, _u16p, ctypes.c_uint64]
             L.wmh_oracle_bound_lw.argtypes = [_f32p, _f32p, ctypes.c_uint64]
+            _f32ref = ctypes.POINTER(ctypes.c_float)
+            L.wmh_oracle_code_range.argtypes = [ctypes.c_float, ctypes.c_float, _f32ref,
+                                                _f32ref]
+            L.wmh_oracle_bound_code.argtypes = [_u16p, _u8p, ctypes.c_uint64, ctypes.c_float,
+                                                ctypes.c_float]
+            L.wmh_oracle_code_threshold.argtypes = [_f32p, _u32p, ctypes.c_uint64,
+                                                    ctypes.c_float, ctypes.c_float]
+            L.wmh_oracle_code_lw.argtypes = [_f32p, _u32p, ctypes.c_uint64, ctypes.c_float]
             L.wmh_oracle_gen_params.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_uint32, _f32p, _f32p, _f32p]
             L.wmh_oracle_gen_params_cols.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
@@ -201,6 +210,38 @@ def bound_lw(lw):
     out = np.empty_like(lw)
     lib().wmh_oracle_bound_lw(lw, out, lw.size)
     return out
+
+
+def code_range(mn, mx):
+    """(lo, delta) of the byte codes of a table with entries in [mn, mx]
+    (wmh_bound.h)."""
+    lo, delta = ctypes.c_float(0.0), ctypes.c_float(0.0)
+    lib().wmh_oracle_code_range(float(mn), float(mx), ctypes.byref(lo), ctypes.byref(delta))
+    return np.float32(lo.value), np.float32(delta.value)
+
+
+def bound_code(h, lo, delta):
+    """Byte codes of bound entries (fp16 bits) for (lo, delta)."""
+    h = np.ascontiguousarray(h, dtype=np.uint16).ravel()
+    c = np.empty(h.size, np.uint8)
+    lib().wmh_oracle_bound_code(h, c, h.size, float(lo), float(delta))
+    return c
+
+
+def code_threshold(thr, scale, t0):
+    """T' (wmh_code_threshold) of running minima thr."""
+    thr = np.ascontiguousarray(thr, dtype=np.float32).ravel()
+    t = np.empty(thr.size, np.uint32)
+    lib().wmh_oracle_code_threshold(thr, t, thr.size, float(scale), float(t0))
+    return t
+
+
+def code_lw(lwb, scale):
+    """0x4000 - cB (wmh_code_lw) of nonzero terms lwb."""
+    lwb = np.ascontiguousarray(lwb, dtype=np.float32).ravel()
+    b = np.empty(lwb.size, np.uint32)
+    lib().wmh_oracle_code_lw(lwb, b, lwb.size, float(scale))
+    return b
 
 
 def philox(ctr, key):

@@ -5,19 +5,27 @@
 //
 //   bound_check [variant] [trials]
 //
-// Counts trials where the bound L = X - lwb exceeds the computed ln_a (a
-// skip that could drop a selectable nonzero).  Variant 0 is the bound the
-// engine uses and must give no violation; the planted mistakes must give
-// some, which shows the trials reach the cases that matter:
+// Variants 0-4 count trials where the bound L = X - lwb exceeds the computed
+// ln_a (a skip that could drop a selectable nonzero).  Variant 0 is the
+// bound the engine uses and must give no violation; the planted mistakes
+// must give some, which shows the trials reach the cases that matter:
 //   1: the "- r" term dropped from the table entry;
 //   2: the sign of lw flipped;
 //   3: no margins, table entry rounded to nearest instead of down;
 //   4: no margins (table entry rounded down, lwb = lw): only the binary32
 //      rounding of ln_a below lc - r - lw, a few ulps, remains to be caught.
+// Variants 5-8 test the byte codes the engine stores (wmh_code_*): over a
+// table range around the entry and running minima thr at or just above
+// ln_a (ties included), a violation is a skip (c >= T + cB) with
+// ln_a <= thr.  Variant 5 is the engine's test and must give none; planted:
+//   6: T without its + 1 (floor((thr - lo) / delta));
+//   7: cB rounded down instead of up;
+//   8: codes rounded up instead of down.
 // Inputs: r, lc, beta and lw drawn over the bound's whole range (generated
 // table distributions, log-uniform extremes, tiny and large |lw|), half of
 // them with beta placed so that lw / r + beta lands on or just above an
 // integer, where ln_a meets its lower end lc - r - lw.
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -89,6 +97,7 @@ int main(int argc, char **argv) {
       beta = (float)b;
       if (beta >= 1.0f || beta < 0.0f) beta = 0.0f;
     }
+    if (variant >= 5 && unit(&s) < 0.02) lw = INFINITY;  // ln_a = -inf
     const float ln_a = ln_a_of(lw, r, beta, lc);
     uint16_t x16;
     float lwb;
@@ -121,11 +130,46 @@ int main(int argc, char **argv) {
         lwb = wmh_bound_lw(lw);
         break;
     }
-    const float L = wmh_f16_to_f32(x16) - lwb;
-    if (L > ln_a) violations++;
+    if (variant < 5) {
+      const float L = wmh_f16_to_f32(x16) - lwb;
+      if (L > ln_a) violations++;
+      if (ln_a - (lc - r - lw) <= 0x1p-10f * (fabsf(lc) + r + fabsf(lw))) near_low_end++;
+      continue;
+    }
+    // byte codes over a table range [mn, mx] holding X (an entry of -inf
+    // leaves the bound off)
+    const float x = wmh_f16_to_f32(x16);
+    if (!(x > -INFINITY) || !(ln_a == ln_a)) continue;
+    const float mn = unit(&s) < 0.2 ? x : x - (float)(unit(&s) * 64.0);
+    const float mx = unit(&s) < 0.2 ? x : x + (float)(unit(&s) * 64.0);
+    float lo, delta;
+    wmh_code_range(mn, mx, &lo, &delta);
+    const float scale = 1.0f / delta;
+    uint32_t code = wmh_bound_code(x16, lo, delta, scale);
+    if (variant == 8) {
+      const float q = ceilf((x - lo) * scale);
+      code = q > 255.0f ? 255u : (uint32_t)q;
+    }
+    // thr: ln_a itself (a tie), the next float up, a random step above, or
+    // the grid point of the codes at or above ln_a
+    float thr = ln_a;
+    const double how = unit(&s);
+    if (how < 0.25) thr = nextafterf(ln_a, INFINITY);
+    else if (how < 0.5) thr = ln_a + (float)(unit(&s) * 2.0) * delta;
+    else if (how < 0.75 && ln_a > -INFINITY) thr = lo + ceilf((ln_a - lo) * scale) * delta;
+    if (!(thr >= ln_a)) thr = ln_a;
+    if (!(thr < FLT_MAX)) continue;  // running minima are below FLT_MAX
+    const float t0 = 2049.0f - lo * scale;
+    const uint32_t tp = wmh_code_threshold(thr, scale, variant == 6 ? t0 - 1.0f : t0);
+    uint32_t b = wmh_code_lw(lwb, scale);
+    if (variant == 7 && lwb > -INFINITY && lwb < INFINITY) {
+      const float q = floorf(lwb * scale);
+      b = (uint32_t)(0x4000 - (int32_t)q);
+    }
+    if (wmh_code_skips(code, tp, b) && !(ln_a > thr)) violations++;
     if (ln_a - (lc - r - lw) <= 0x1p-10f * (fabsf(lc) + r + fabsf(lw))) near_low_end++;
   }
   printf("variant %d trials %lld violations %lld near_low_end %lld\n", variant, trials,
          violations, near_low_end);
-  return variant == 0 && violations != 0 ? 1 : 0;
+  return (variant == 0 || variant == 5) && violations != 0 ? 1 : 0;
 }

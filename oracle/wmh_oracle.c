@@ -57,6 +57,18 @@ void wmh_oracle_bound_entry(const float *r, const float *lc, uint16_t *h, uint64
 void wmh_oracle_bound_lw(const float *lw, float *out, uint64_t n) {
   for (uint64_t i = 0; i < n; i++) out[i] = wmh_bound_lw(lw[i]);
 }
+void wmh_oracle_code_range(float mn, float mx, float *lo, float *delta) {
+  wmh_code_range(mn, mx, lo, delta);
+}
+void wmh_oracle_bound_code(const uint16_t *h, uint8_t *c, uint64_t n, float lo, float delta) {
+  for (uint64_t i = 0; i < n; i++) c[i] = (uint8_t)wmh_bound_code(h[i], lo, delta, 1.0f / delta);
+}
+void wmh_oracle_code_threshold(const float *thr, uint32_t *t, uint64_t n, float scale, float t0) {
+  for (uint64_t i = 0; i < n; i++) t[i] = wmh_code_threshold(thr[i], scale, t0);
+}
+void wmh_oracle_code_lw(const float *lwb, uint32_t *b, uint64_t n, float scale) {
+  for (uint64_t i = 0; i < n; i++) b[i] = wmh_code_lw(lwb[i], scale);
+}
 
 void wmh_oracle_logf_vec(const float *x, float *y, uint64_t n) {
   for (uint64_t i = 0; i < n; i++) y[i] = wmh_logf(x[i]);

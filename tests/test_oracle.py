@@ -369,12 +369,15 @@ def test_bound_table_and_lw_terms():
 
 
 @pytest.mark.parametrize("variant,expect_violations", [(0, False), (1, True), (2, True),
-                                                       (3, True), (4, True)])
+                                                       (3, True), (4, True), (5, False),
+                                                       (6, True), (7, True), (8, True)])
 def test_skip_bound_brute_force(variant, expect_violations):
     """The skip bound never exceeds the computed ln_a over 2e7 adversarial
     trials (oracle/bound_check.c); each planted mistake -- the r term dropped,
     the sign of lw flipped, a nearest-rounded entry, no binary32 margin --
-    is caught."""
+    is caught.  Variants 5-8: the byte-code test never skips a (nonzero,
+    sample) whose ln_a is at or below the running minimum (ties included);
+    T without its + 1, cB rounded down and codes rounded up are caught."""
     import subprocess
     res = subprocess.run([oracle.BOUND_CHECK, str(variant), "20000000"], capture_output=True,
                          text=True, timeout=300)
@@ -383,6 +386,83 @@ def test_skip_bound_brute_force(variant, expect_violations):
         assert violations > 0, res.stdout
     else:
         assert res.returncode == 0 and violations == 0, res.stdout
+
+
+def test_code_sum_bit15_is_the_skip_test():
+    """Bit 15 of c + (0x4800 - T') + (0x4000 - cB) is set exactly when
+    c >= T + cB (T = T' - 2048) over every code, every T' and the whole
+    clamped cB range, and the packed form (two 16-bit halves in one 32-bit
+    add) gives each half the same answer: no carry crosses the halves."""
+    c = np.arange(256, dtype=np.int64)[:, None, None]
+    tp = np.arange(0, 4352, dtype=np.int64)[None, :, None]
+    cb = np.concatenate([np.arange(-2048, 2305, 37), [-2048, -1653, 0, 1420, 2304]])
+    cb = cb.astype(np.int64)[None, None, :]
+    h = c + (0x4800 - tp) + (0x4000 - cb)
+    assert h.min() >= 0x6E01 and h.max() <= 0x90FF
+    assert np.array_equal((h >> 15) & 1, (c >= tp - 2048 + cb).astype(np.int64))
+    rng = np.random.default_rng(5)
+    n = 1_000_000
+    c2 = rng.integers(0, 256, (2, n))
+    a2 = 0x4800 - rng.integers(0, 4352, (2, n))
+    b = 0x4000 - rng.integers(-2048, 2305, n)
+    word = (c2[0] | c2[1] << 16) + (a2[0] | a2[1] << 16) + (b | b << 16)
+    for half in (0, 1):
+        single = c2[half] + a2[half] + b
+        assert np.array_equal((word >> (16 * half + 15)) & 1, (single >> 15) & 1)
+
+
+def test_codes_are_lower_bounds_of_the_entries():
+    """Codes of generated-table entries and of every fp16 pattern decode to
+    values at or below the entry, within one step of it unless clamped at
+    255; the range covers the table with lo a multiple of delta >= 1/16."""
+    rs, ln_cs, _ = oracle.gen_params(4, 3000, 128)
+    h = oracle.bound_entry(rs, ln_cs)
+    x = oracle.f16_to_f32(h).astype(np.float64)
+    lo, delta = oracle.code_range(x.min(), x.max())
+    assert delta >= 1 / 16 and lo <= x.min() and float(lo / delta).is_integer()
+    assert 255 * delta >= 0.9 * (x.max() - x.min())
+    c = oracle.bound_code(h, lo, delta).astype(np.float64)
+    v = float(lo) + c * float(delta)
+    assert np.all(v <= x)
+    assert np.all((v + float(delta) > x) | (c == 255))
+    # every finite fp16 value above a range's lower end
+    allh = np.arange(1 << 16, dtype=np.uint32).astype(np.uint16)
+    allx = oracle.f16_to_f32(allh).astype(np.float64)
+    for mn, mx in ((-30.0, 3.0), (-70000.0, 70000.0), (-1.0, -1.0), (5.5, 9.25)):
+        lo, delta = oracle.code_range(mn, mx)
+        ok = np.isfinite(allx) & (allx >= lo)
+        cc = oracle.bound_code(allh[ok], lo, delta).astype(np.float64)
+        vv = float(lo) + cc * float(delta)
+        assert np.all(vv <= allx[ok])
+        assert np.all((vv + float(delta) > allx[ok]) | (cc == 255))
+
+
+def test_code_thresholds_and_lw_terms():
+    """T' is never below the exact floor((thr - lo) / delta) + 2049 (clamped
+    to [0, 4351]) and at most one above it (the fma's rounding may reach the
+    next integer: more evaluation, never less), exact on the code grid;
+    0x4000 - cB equals 0x4000 - ceil(lwb / delta) for finite lwb; the
+    infinities take the clamps that evaluate them."""
+    rng = np.random.default_rng(6)
+    for mn, mx in ((-30.0, 3.0), (-500.0, 40.0)):
+        lo, delta = oracle.code_range(mn, mx)
+        scale, t0 = np.float32(1 / delta), np.float32(2049 - lo / delta)
+        thr = np.concatenate([rng.uniform(-600, 600, 400_000),
+                              float(lo) + rng.integers(-3000, 3000, 100_000) * float(delta),
+                              [np.inf, -np.inf, 3.4e38, -3.4e38]]).astype(np.float32)
+        t = oracle.code_threshold(thr, scale, t0).astype(np.float64)
+        with np.errstate(invalid="ignore", over="ignore"):
+            exact = np.floor((thr.astype(np.float64) - float(lo)) / float(delta)) + 2049
+        want = np.clip(exact, 0, 4351)
+        assert np.all(t >= want) and np.all(t <= want + 1)
+        grid = slice(400_000, 500_000)
+        assert np.array_equal(t[grid], want[grid])
+        lw = np.concatenate([rng.uniform(-103.3, 88.7, 400_000), [0.0, 88.72, -103.28]])
+        lwb = oracle.bound_lw(lw.astype(np.float32))
+        b = oracle.code_lw(lwb, scale).astype(np.int64)
+        assert np.all(b == 0x4000 - np.ceil(lwb.astype(np.float64) / float(delta)))
+        inf = oracle.code_lw(np.float32([np.inf, -np.inf]), scale)
+        assert list(inf) == [0x4000 - 2304, 0x4000 + 2048]
 
 
 def test_fast_cpu_baseline_equals_oracle():
