@@ -75,6 +75,8 @@ def parse(argv=None):
     p.add_argument("--no-skip", action="store_true",
                    help="hash without the skip bound (mhcuda_set_skip(0))")
     p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
+    p.add_argument("--no-seed2", action="store_true",
+                   help="batched: skip the second layout seed")
     p.add_argument("--no-large", action="store_true",
                    help="batched: leave the 1M x 12M share out of the line")
     p.add_argument("--no-serve", action="store_true",
@@ -334,6 +336,33 @@ def autotune(engine, descs, stream):
     tuned = dict(tr.as_dict(), seconds=time.perf_counter() - t0)
     log("autotuned: %s" % tuned)
     return tr.kernel, tuned
+
+
+def second_seed(args, torch, engine, kernel, stream, steps):
+    """The uneven batch again with another draw of its layout (row counts and
+    nnz/row from seed 2, matrix contents from seeds 200 + i), hashed with the
+    engine's current tuning: a check that the headline figure is not a
+    property of one draw.  Returns its value and timings."""
+    from minhashcuda_amd import workloads
+    cm = engine.capi_mod
+    dim = 1 << 20
+    layout = workloads.batched_layout(count=args.count, seed=2)
+    tensors = [workloads.batched_csr_torch(n, dim, 1.5, seed=200 + i)
+               for i, n in enumerate(layout)]
+    lengths = [int(t[2].numel()) - 1 for t in tensors]
+    nnz = sum(int(t[0].numel()) for t in tensors)
+    outs = [torch.empty((n, SAMPLES, 2), dtype=torch.int32, device="cuda") for n in lengths]
+    descs = (cm.Matrix * len(tensors))()
+    for i, ((w, c, r), o) in enumerate(zip(tensors, outs)):
+        descs[i] = cm.Matrix(w.data_ptr(), c.data_ptr(), r.data_ptr(), lengths[i], o.data_ptr())
+    torch.cuda.synchronize()
+    wall, kms = timed_steps(torch, lambda: engine.batch(descs, kernel, stream), engine, steps, 1)
+    res = {"layout_seed": 2, "rows": sum(lengths), "nnz": nnz, "steps": steps,
+           "value": nnz * SAMPLES * steps / wall, "ms_per_step": wall * 1e3 / steps,
+           "avg_kernel_ms": sum(kms) / len(kms), "tuning": engine.tuning()}
+    del tensors, outs, descs
+    torch.cuda.empty_cache()
+    return res
 
 
 def timed_steps(torch, run, engine, steps, warmup):
@@ -811,8 +840,12 @@ def main(argv=None):
             gh = outs[0].cpu().numpy().view(np.uint32)
             result["cpu_baseline"] = cpu_baseline(params, mats[0], SAMPLES, args.cpu_seconds,
                                                   gh, "plain workload")
-    engine.close()
     del tensors, outs, descs
+    torch.cuda.empty_cache()
+    if args.workload == "batched" and not args.no_seed2:
+        log("second layout seed")
+        result["seed2"] = second_seed(args, torch, engine, kernel, stream, min(steps, 3))
+    engine.close()
     torch.cuda.empty_cache()
     if args.workload == "batched" and not args.no_large:
         log("large share")

@@ -880,7 +880,7 @@ __device__ __forceinline__ void code_thresholds(const float (&thr)[8], float sca
 template <bool Fast, int H>
 __device__ __forceinline__ void fold_half(const float4 &r4, const float4 &b4, const float4 &c4,
                                           float lw, uint32_t p, bool on, float (&best)[8],
-                                          uint32_t (&pos)[8], float (&thr)[8]) {
+                                          uint32_t (&pos)[8]) {
   WMH_NO_CONTRACT
   const float r[4] = {r4.x, r4.y, r4.z, r4.w};
   const float beta[4] = {b4.x, b4.y, b4.z, b4.w};
@@ -894,7 +894,6 @@ __device__ __forceinline__ void fold_half(const float4 &r4, const float4 &b4, co
     const bool upd = on && ln_a < best[v];                // kernel.cu:106
     best[v] = upd ? ln_a : best[v];
     pos[v] = upd ? p : pos[v];
-    thr[v] = upd && ln_a < thr[v] ? ln_a : thr[v];
   }
 }
 
@@ -1017,16 +1016,18 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
         const uint32_t ou = bperm_u32(o, src) + 4u * koff;
         if (has) {
           float4 r4 = load4(rs, ou, 0u), b4 = load4(rs, ou, s_b), c4 = load4(rs, ou, s_c);
-          fold_half<kFast, 0>(r4, b4, c4, lws, cb + src, true, best, pos, thr);
+          fold_half<kFast, 0>(r4, b4, c4, lws, cb + src, true, best, pos);
           r4 = load4(rs, ou + 16u, 0u);
           b4 = load4(rs, ou + 16u, s_b);
           c4 = load4(rs, ou + 16u, s_c);
-          fold_half<kFast, 1>(r4, b4, c4, lws, cb + src, true, best, pos, thr);
+          fold_half<kFast, 1>(r4, b4, c4, lws, cb + src, true, best, pos);
         }
       }
     };
     if (any_marked) {
       with_division(fast, [&](auto mode) { phase2(mode); });
+#pragma unroll
+      for (int v = 0; v < 8; v++) thr[v] = best[v] < thr[v] ? best[v] : thr[v];
       code_thresholds(thr, code_scale, code_t0, tpair);
     }
   }
