@@ -122,6 +122,10 @@ def test_committed_lines_keep_the_contract(name):
 def test_default_line_carries_the_large_share_and_serving():
     line = load_line("batched")
     assert line["batched_over_plain"] >= 3.0 and line["plain"]["identical_to_batched"]
+    # a second draw of the batch's layout, hashed with the same tuning
+    seed2 = line["seed2"]
+    assert seed2["layout_seed"] == 2 and seed2["nnz"] != line["config"]["nnz"]
+    assert 0.8 <= seed2["value"] / line["value"] <= 1.25
     large = line["large"]
     assert large["value"] > 0 and 0 < large["roofline"]["frac"] <= 1
     assert large["end_to_end"]["identical_to_resident_chunk"] is True
