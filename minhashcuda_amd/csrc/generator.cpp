@@ -219,7 +219,7 @@ static MHCUDAResult hash_persistent(MinhashCudaGenerator *gen, std::vector<Matri
   HIPCH(launch_persistent(gen->params, gen->dim, gen->samples, (MatrixDesc *)dev, count,
                           (uint32_t)chunks, (BatchCtl *)(dev + ctl_off),
                           (StripedRow *)(dev + layout.list_off), gen->tuning, gen->seg_auto,
-                          gen->num_cus, gen->fast_div, s,
+                          gen->num_cus, gen->num_xcds, gen->fast_div, s,
                           with_events ? gen->ev_begin : nullptr,
                           with_events ? gen->ev_end : nullptr, launch_bounds_of(gen)),
         mhcudaRuntimeError);
@@ -432,6 +432,15 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
   hipDeviceProp_t props;
   if (hipGetDeviceProperties(&props, dev) != hipSuccess) FAIL(mhcudaRuntimeError);
   gen->num_cus = props.multiProcessorCount;
+  // XCDs (dies with an L2 of their own): one stripe queue per XCD; 1 where the
+  // runtime does not say
+  int xcds = 1;
+  if (hipDeviceGetAttribute(&xcds, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess ||
+      xcds < 1) {
+    (void)hipGetLastError();
+    xcds = 1;
+  }
+  gen->num_xcds = xcds;
   if (hipStreamCreateWithFlags(&gen->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&gen->copy_stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&gen->d2h_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -600,8 +609,8 @@ MHCUDAResult mhcuda_calc_device_noalloc(const MinhashCudaGenerator *cgen, const 
                                reinterpret_cast<wmh::MatrixDesc *>(scr), 1u, (length + 63u) / 64u,
                                reinterpret_cast<wmh::BatchCtl *>(scr + wmh::kScratchCtlOff),
                                reinterpret_cast<wmh::StripedRow *>(scr + wmh::kScratchListOff),
-                               gen->tuning, gen->seg_auto, gen->num_cus, gen->fast_div, s,
-                               nullptr, nullptr, wmh::launch_bounds_of(gen)),
+                               gen->tuning, gen->seg_auto, gen->num_cus, gen->num_xcds,
+                               gen->fast_div, s, nullptr, nullptr, wmh::launch_bounds_of(gen)),
         mhcudaRuntimeError);
   if (!stream) HIPCH(hipStreamSynchronize(s), mhcudaRuntimeError);
   return mhcudaSuccess;

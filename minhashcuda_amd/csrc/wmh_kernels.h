@@ -30,11 +30,12 @@ struct MatrixDesc {
 };
 
 // Control block of one persistent launch (device memory).
-// Work queues of the persistent kernel: one per XCD for stripe pieces (the
-// stripes k with k % kXcds == q go to queue q), then one for the items.
-constexpr uint32_t kXcds = 8;
+// Work queues of the persistent kernel: up to one per XCD for stripe pieces
+// (with Q of them, the stripes k with k % Q == q go to queue q), then the
+// items' queue at index kMaxXcds.
+constexpr uint32_t kMaxXcds = 8;
 struct BatchCtl {
-  unsigned long long next[kXcds + 1];  // queue heads
+  unsigned long long next[kMaxXcds + 1];  // queue heads
   unsigned long long total_units;      // items * sample tiles
   uint32_t total_items;   // below 2^31 + count (see batch_plan_kernel)
   uint32_t ntiles;
@@ -74,9 +75,10 @@ struct StripePlan {
   uint32_t stripes;    // 1: no striping
   uint32_t min_nnz;    // rows with at least this many nonzeros are striped
   uint32_t per_unit;   // pieces handed out per queue unit
-  uint32_t queues;     // stripe queues: kXcds (one per XCD) or 1
+  uint32_t queues;     // stripe queues: `xcds` (one per XCD) or 1
   uint32_t width;      // columns per stripe: ceil(dim / stripes)
   StripedRow *list;    // capacity: every row of the batch
+  uint32_t xcds;       // the device's XCDs (workgroups go to them round-robin)
 };
 // Stripes up to this size get one queue per XCD (each XCD's L2 holds 4 MiB).
 constexpr uint64_t kXcdStripeBytes = 8ull << 20;
@@ -186,7 +188,8 @@ hipError_t launch_plain(const float *params, uint32_t dim, uint32_t samples,
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
-                             bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
+                             bool auto_seg, int num_cus, int num_xcds, bool fast_div,
+                             hipStream_t stream,
                              hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
                              CodeTable table = CodeTable{nullptr, 0.0f, 0.0f});
 

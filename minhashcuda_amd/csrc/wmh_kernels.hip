@@ -1083,12 +1083,12 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
   const uint32_t ntiles = uniform(ctl->ntiles);
-  // Stripe pieces go through Q = stripes.queues queues (1 or kXcds): queue q
-  // holds the stripes k with k % Q == q, in increasing k, a unit being
+  // Stripe pieces go through Q = stripes.queues queues (1 or one per XCD):
+  // queue q holds the stripes k with k % Q == q, in increasing k, a unit being
   // `per_unit` consecutive pieces of one stripe.  Workgroups are dispatched to
-  // the XCDs round-robin, so with Q = kXcds a wave of workgroup w serves queue
-  // w % kXcds first: each XCD then gathers from its own stripe, which its L2
-  // can hold, rather than every XCD caching the same one (stripes larger than
+  // the XCDs round-robin, so with one queue per XCD a wave of workgroup w
+  // serves queue w % Q first: each XCD then gathers from its own stripe, which
+  // its L2 can hold, rather than every XCD caching the same one (stripes larger than
   // an L2 are better shared, Q = 1, so that together they stay in the
   // Infinity Cache).  A wave then takes items of the other rows from the
   // shared queue, and finally helps drain the other stripe queues, so every
@@ -1120,20 +1120,20 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   const bool items_first =
       V == 2 && K != 0 &&
       uniform_u64(ctl->striped_nnz) * 4ull >= uniform_u64(ctl->total_nnz) * 3ull &&
-      (blockIdx.x / kXcds) % kItemsFirstEvery == 0;
+      (blockIdx.x / stripes.xcds) % kItemsFirstEvery == 0;
   for (uint32_t step = 0; step <= Q; step++) {
-    const uint32_t qid = step == 0 ? (items_first ? kXcds : home)
-                                   : (step == 1 ? (items_first ? home : kXcds)
+    const uint32_t qid = step == 0 ? (items_first ? kMaxXcds : home)
+                                   : (step == 1 ? (items_first ? home : kMaxXcds)
                                                 : (home + step - 1) % Q);
     const uint32_t nk = qid < Q && qid < K ? (K - qid + Q - 1) / Q : 0u;
-    const uint64_t qtotal = qid == kXcds ? item_units : per_stripe * nk;
+    const uint64_t qtotal = qid == kMaxXcds ? item_units : per_stripe * nk;
     if (qtotal == 0) continue;
     for (;;) {
       uint64_t u = 0;
       if (lane == 0) u = atomicAdd(&ctl->next[qid], 1ull);
       u = ((uint64_t)readlane_u32((uint32_t)(u >> 32), 0) << 32) | readlane_u32((uint32_t)u, 0);
       if (u >= qtotal) break;  // this queue is drained
-      const bool piece = qid < kXcds;
+      const bool piece = qid < kMaxXcds;
       uint32_t k = 0;
       uint64_t q0 = 0, q_end = 1;
       // lane l holds the row of the unit's piece l and the piece's bounds
@@ -1284,7 +1284,7 @@ __global__ __launch_bounds__(kBlock) void batch_plan_kernel(MatrixDesc *__restri
     if (tid == kBlock - 1) carry += part[kBlock - 1];
     __syncthreads();
   }
-  if (tid <= kXcds) ctl->next[tid] = 0;
+  if (tid <= kMaxXcds) ctl->next[tid] = 0;
   if (tid == 0) {
     ctl->n_striped = 0;
     ctl->total_nnz = total;
@@ -1633,20 +1633,22 @@ hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t tot
 hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples,
                              MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                              BatchCtl *ctl, StripedRow *striped, const Tuning &tuning,
-                             bool auto_seg, int num_cus, bool fast_div, hipStream_t stream,
-                             hipEvent_t ev_begin, hipEvent_t ev_end, CodeTable table) {
+                             bool auto_seg, int num_cus, int num_xcds, bool fast_div,
+                             hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end,
+                             CodeTable table) {
   const uint32_t sp = padded_samples(samples);
   // the bounded pieces need 8 whole samples per lane
   const uint8_t *codes = samples % 8u == 0u ? table.codes : nullptr;
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;
-  StripePlan stripes{1u, 0xffffffffu, 1u, 1u, 0u, striped};
+  const uint32_t xcds = (uint32_t)std::min<int>(std::max(num_xcds, 1), (int)kMaxXcds);
+  StripePlan stripes{1u, 0xffffffffu, 1u, 1u, 0u, striped, xcds};
   stripe_shape(dim, samples, tuning.stripe_mib, &stripes.stripes, &stripes.min_nnz);
   stripes.width = (uint32_t)(((uint64_t)dim + stripes.stripes - 1) / stripes.stripes);
   // stripes that fit an XCD's L2 (twice over) get one queue per XCD
   const uint64_t stripe_bytes = (uint64_t)dim * 12u * sp / stripes.stripes;
-  stripes.queues = stripe_bytes <= kXcdStripeBytes ? kXcds : 1u;
+  stripes.queues = stripe_bytes <= kXcdStripeBytes ? xcds : 1u;
   // about half an item's nonzeros of the shortest pieces per queue unit (at
   // most one per lane: a unit's pieces are read with one vector load)
   stripes.per_unit =

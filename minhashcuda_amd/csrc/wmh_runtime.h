@@ -162,6 +162,7 @@ struct MinhashCudaGenerator_ {
   int device = -1;
   uint32_t sp = 0;
   int num_cus = 0;
+  int num_xcds = 1;
   float *params = nullptr;            // [dim][3][sp]
   // where the tables came from: nothing yet (deferred init), the on-device
   // generator with `seed`, or host tables (assign / a parameter file)

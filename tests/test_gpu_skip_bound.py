@@ -62,8 +62,11 @@ def test_skip_on_and_off_equal_the_oracle(libmhcuda, capi, gpu_available, sample
 
 
 def test_tables_outside_the_bound_range(libmhcuda, capi, gpu_available):
-    """Assigned tables with a beta of 1, an infinite ln_c or a huge ln_c turn
-    the bound off (and the hashes stay exact); valid tables turn it on."""
+    """Assigned tables with a beta of 1, an infinite ln_c, a huge ln_c or an
+    ln_c - r below the fp16 range (a bound entry of -inf: the codes have no
+    finite lower end) turn the bound off (and the hashes stay exact); valid
+    tables turn it on, also with a range of bound entries so wide that the
+    code step is 256 (the codes then prune little but stay lower bounds)."""
     dim, samples = 20000, 128
     mats = long_row_batch(dim, 3)
     rs, ln_cs, betas = oracle.datasketch_style_params(dim, samples, seed=4)
@@ -83,6 +86,9 @@ def test_tables_outside_the_bound_range(libmhcuda, capi, gpu_available):
         c3 = ln_cs.copy()
         c3[0, 5] = np.float32(2.0 ** 61)
         cases.append((rs, c3, betas))
+        c4 = ln_cs.copy()
+        c4[5, 9] = np.float32(-1e5)
+        cases.append((rs, c4, betas))
         for params in cases:
             libmhcuda.minhash_cuda_assign_vars(gen, *params)
             assert skip_state(capi, g) == (1, 0)
@@ -90,12 +96,15 @@ def test_tables_outside_the_bound_range(libmhcuda, capi, gpu_available):
             for m, h in zip(mats, got):
                 assert np.array_equal(h, oracle.calc_omp(*params, m.data, m.indices,
                                                          m.indptr, 16)[0])
-        libmhcuda.minhash_cuda_assign_vars(gen, rs, ln_cs, betas)
-        assert skip_state(capi, g) == (1, 1)
-        got = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
-        for m, h in zip(mats, got):
-            assert np.array_equal(h, oracle.calc_omp(rs, ln_cs, betas, m.data, m.indices,
-                                                     m.indptr, 16)[0])
+        c5 = ln_cs.copy()
+        c5[1, 17], c5[2, 40] = np.float32(-3e4), np.float32(3e4)
+        for lc in (ln_cs, c5):
+            libmhcuda.minhash_cuda_assign_vars(gen, rs, lc, betas)
+            assert skip_state(capi, g) == (1, 1)
+            got = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
+            for m, h in zip(mats, got):
+                assert np.array_equal(h, oracle.calc_omp(rs, lc, betas, m.data, m.indices,
+                                                         m.indptr, 16)[0])
     finally:
         libmhcuda.minhash_cuda_fini(gen)
 
