@@ -52,13 +52,12 @@ def candidate_pairs_gpu(gen, keys, max_bucket=64, lib=None):
     of mhcuda_lsh_band_keys.  Returns the same int64 array [n][2] (i < j,
     sorted) as candidate_pairs."""
     import ctypes
-    import os
 
     import torch
 
+    from . import LIB_PATH
     from . import capi as capi_mod
-    lib = lib or capi_mod.load(os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                                            "libMHCUDA.so"))
+    lib = lib or capi_mod.load(LIB_PATH)
     rows, bands = keys.shape
     keys = keys.contiguous()
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

@@ -10,14 +10,12 @@ Requests are hashed by continuous batching: whatever is pending when a slot
 frees up goes into one persistent launch; slots overlap on the device.
 """
 import ctypes
-import os
 import threading
 
 import numpy as np
 
+from . import LIB_PATH
 from . import capi as capi_mod
-
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libMHCUDA.so")
 
 
 class Server:
