@@ -170,16 +170,17 @@ MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib);
 MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib,
                                 uint32_t *stripes, uint32_t *min_nnz);
 
-/* The skip bound of the persistent kernel's column-stripe pieces (on by
- * default): a table of one byte per (column, sample), a code of a lower
- * bound of the hash value, lets a piece skip the exact evaluation of every
- * (nonzero, sample) that provably cannot beat the row's current minimum.
- * The hashes are the same either way.  It needs a sample count divisible by
- * 8, tables with every r in [2^-60, 2^60], beta in [0, 1) and
- * |ln_c| <= 2^60 whose every |ln_c - r| stays within the fp16 range
+/* The skip bound of the persistent kernel (on by default): a table of one
+ * byte per (column, sample), a code of a lower bound of the hash value, lets
+ * the kernel skip the exact evaluation (and the parameter gathers) of every
+ * (nonzero, sample) that provably cannot beat the row's current minimum, in
+ * its row walks and in its column-stripe pieces.  The hashes are the same
+ * either way.  It needs tables with every r in [2^-60, 2^60], beta in
+ * [0, 1) and |ln_c| <= 2^60 whose every |ln_c - r| stays within the fp16 range
  * (generated tables always qualify; assigned or loaded ones are checked),
  * and dim * roundup(samples, 4) bytes of device memory (allocated by
- * mhcuda_init; MHCUDA_SKIP_BOUND=0 in the environment skips that).
+ * mhcuda_init; MHCUDA_SKIP_BOUND=0 in the environment skips that); the
+ * stripe pieces also need a sample count divisible by 8.
  * mhcuda_set_skip(0) turns it off; mhcuda_get_skip reports the switch and
  * whether the bound is in use for the current tables. */
 MHCUDAResult mhcuda_set_skip(MinhashCudaGenerator *gen, int enable);

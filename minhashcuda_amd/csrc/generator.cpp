@@ -937,7 +937,7 @@ MHCUDAResult mhcuda_set_skip(MinhashCudaGenerator *gen, int enable) {
 MHCUDAResult mhcuda_get_skip(const MinhashCudaGenerator *gen, int *enabled, int *active) {
   if (!gen) return mhcudaInvalidArguments;
   if (enabled) *enabled = gen->skip ? 1 : 0;
-  if (active) *active = wmh::launch_bounds_of(gen).codes != nullptr && gen->samples % 8 == 0;
+  if (active) *active = wmh::launch_bounds_of(gen).codes != nullptr;
   return mhcudaSuccess;
 }
 

@@ -540,6 +540,73 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
   }
 }
 
+// Skip bound inside a row walk (wmh_bound.h; the byte-coded bound table is
+// laid out [dim][sp]).  A lane holds V samples k0 .. k0 + V - 1 of every
+// nonzero, so its V codes of one nonzero are one V-byte load.  Per block of
+// kCodeGroups groups of positions the lane tests each (nonzero, sample) against the
+// threshold T' of its running minimum for that sample and marks the nonzeros
+// for which some of its samples are not skipped; only marked (nonzero, lane)
+// pairs gather parameters and fold.  An unmarked pair's computed ln_a is
+// strictly above the lane's running minimum at the start of the block, so it
+// can never be selected (strict <, ties keep the earlier position) -- nor win
+// the keyed merge of a split row, whose minimum it is compared with belongs to
+// a real candidate of the same row.  Positions past the end of the row that
+// is current at the start of the block are marked for every lane.
+// (one group per block keeps the hashing kernels within 128 VGPRs, 4 waves
+// per SIMD; two groups need 141 at 4 samples per lane)
+#ifndef WMH_CODE_GROUPS
+#define WMH_CODE_GROUPS 1
+#endif
+constexpr uint32_t kCodeGroups = WMH_CODE_GROUPS;
+// Buffer offset past every column block (block sizes are far below 2^31):
+// an unmarked lane's gather returns zero and reads no memory.
+constexpr uint32_t kNoGather = 0x80000000u;
+
+// Packed 0x4800 - T' of the lane's V running minima, two samples per word
+// (wmh_bound.h).  At one sample per lane the high half is never tested.
+template <int V>
+__device__ __forceinline__ void lane_thresholds(const float (&best)[V], float scale, float t0,
+                                                uint32_t (&tp)[(V + 1) / 2]) {
+#pragma unroll
+  for (int q = 0; q < (V + 1) / 2; q++) {
+    const uint32_t lo16 = 0x4800u - wmh_code_threshold(best[2 * q], scale, t0);
+    const uint32_t hi16 =
+        2 * q + 1 < V ? 0x4800u - wmh_code_threshold(best[(2 * q + 1) % V], scale, t0) : 0u;
+    tp[q] = lo16 | (hi16 << 16);
+  }
+}
+
+// The lane's V codes of one nonzero (bytes of a b32 / b16 / b8 load).
+template <int V>
+__device__ __forceinline__ uint32_t load_codes(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  if constexpr (V == 4) return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+  else if constexpr (V == 2) return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rs, off, 0, 0);
+  else return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(rs, off, 0, 0);
+}
+
+// 1 when some of the lane's V samples is not skipped for the nonzero whose
+// codes are x and whose term (0x4000 - cB, both halves) is lw_pair.
+template <int V>
+__device__ __forceinline__ bool codes_mark(uint32_t x, const uint32_t (&tp)[(V + 1) / 2],
+                                           uint32_t lw_pair) {
+  uint32_t h;
+  if constexpr (V == 4) {
+    h = (__builtin_amdgcn_perm(0u, x, 0x0c010c00u) + tp[0] + lw_pair) &
+        (__builtin_amdgcn_perm(0u, x, 0x0c030c02u) + tp[1] + lw_pair);
+  } else if constexpr (V == 2) {
+    h = __builtin_amdgcn_perm(0u, x, 0x0c010c00u) + tp[0] + lw_pair;
+  } else {
+    h = (x + tp[0] + lw_pair) | 0x80000000u;
+  }
+  return (h & 0x80008000u) != 0x80008000u;
+}
+
+// The code table and its scale (1 / delta) and t0 (wmh_code_threshold).
+struct BoundCodes {
+  const uint8_t *codes;
+  float scale, t0;
+};
+
 // Hashes CSR positions [a, b) of matrix M (one item or one stripe piece) for
 // the lane's samples.  In a split row the argmin tracks the CSR position
 // instead of the column.  `row` is the row containing a, or ~0u to search
@@ -548,15 +615,19 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
 // Unless skip_nnz is 0, rows with at least `skip_nnz` nonzeros are skipped
 // (their stripe pieces hash them); skip_nnz >= 64.  `streaming` gathers with
 // non-temporal loads (items hashed while other wavefronts work through a
-// cache-resident stripe).
-template <int V, int U>
+// cache-resident stripe).  Bounded: the skip bound above, on bc's codes.
+template <int V, int U, bool Bounded>
 __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint32_t dim,
                                           uint32_t sp, uint32_t samples, uint32_t k0,
                                           LaneOffsets lo, const MatrixDesc &M,
                                           uint32_t a, uint32_t b, uint32_t row,
                                           uint32_t piece_start, uint32_t piece_end,
-                                          uint32_t skip_nnz, bool fast_div, bool streaming) {
+                                          uint32_t skip_nnz, bool fast_div, bool streaming,
+                                          BoundCodes bc) {
   static_assert(kWave % U == 0, "groups tile a 64-nonzero chunk");
+  // positions whose codes are loaded together
+  constexpr uint32_t CB = kCodeGroups * (uint32_t)U;
+  static_assert(kWave % CB == 0, "blocks tile a chunk");
   const uint32_t lane = lane_id();
   const char *pbase = reinterpret_cast<const char *>(params);
   const uint64_t block_bytes = 12ull * sp;
@@ -564,6 +635,9 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
   const bool in_row = piece_end != 0u;
+  // the lane's codes sit at byte koff of a column's sp-byte run
+  const uint32_t koff = k0 < samples ? k0 : 0u;
+  const bool lane_ok = k0 < samples;
   uint32_t r = row != ~0u ? row : find_row(rows, M.length, a);
   uint32_t row_start = in_row ? piece_start : uniform(rows[r]);
   // rows[rp_base + lane]: the next 64 row ends, read with one vector load
@@ -612,6 +686,9 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
       c = cols[cb + lane];
       lw = wmh_logf(weights[cb + lane]);
     }
+    // the nonzero's term of the bound test, 0x4000 - cB (lane = nonzero)
+    uint32_t lwt = 0u;
+    if constexpr (Bounded) lwt = wmh_code_lw(wmh_bound_lw(lw), bc.scale);
     const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
     // As in hash_range, the last group may run past n (slots >= n hold the
     // column WMH_SENTINEL and are never selected); only real positions can
@@ -620,36 +697,69 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
     auto groups = [&](auto mode, auto stream) {
       constexpr bool kFast = decltype(mode)::value;
       constexpr bool kStream = decltype(stream)::value;
-      for (uint32_t i = 0; i < n && !skip; i += U) {
-        uint32_t j[U];
-        float lwu[U];
-        float rr[U][V], beta[U][V], lc[U][V];
+      for (uint32_t i0 = 0; i0 < n && !skip; i0 += CB) {
+        // bit u: this lane folds position cb + i0 + u
+        uint32_t marks = 0xffffffffu;
+        if constexpr (Bounded) {
+          uint32_t tp[(V + 1) / 2];
+          lane_thresholds<V>(acc.best, bc.scale, bc.t0, tp);
+          uint32_t x[CB];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          j[u] = readlane_u32(c, i + u);
-          lwu[u] = readlane_f32(lw, i + u);
-          const __amdgpu_buffer_rsrc_t rs = block_rsrc(
-              pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
-          load_b<V, kStream>(rs, lo.r, rr[u]);
-          load_b<V, kStream>(rs, lo.beta, beta[u]);
-          load_b<V, kStream>(rs, lo.lc, lc[u]);
+          for (int u = 0; u < (int)CB; u++) {
+            const uint32_t j = readlane_u32(c, i0 + u);
+            const __amdgpu_buffer_rsrc_t rs =
+                block_rsrc(reinterpret_cast<const char *>(bc.codes) +
+                               (uint64_t)(j < dim ? j : 0u) * sp, sp);
+            x[u] = load_codes<V>(rs, koff);
+          }
+          marks = 0u;
+#pragma unroll
+          for (int u = 0; u < (int)CB; u++) {
+            const uint32_t src = i0 + u;
+            const uint32_t lt = readlane_u32(lwt, src);
+            const uint32_t j = readlane_u32(c, src);
+            // past the current row: another row's minima do not apply
+            const bool other_row = cb + src >= row_end;
+            const bool ev = j < dim && lane_ok &&
+                            (other_row || codes_mark<V>(x[u], tp, lt | (lt << 16)));
+            marks |= ev ? 1u << u : 0u;
+          }
         }
-        const uint32_t p0 = cb + i;
-        if (row_end > p0 + min((uint32_t)U, n - i)) {
-          // the current row runs past the group: fold it without row checks
-#pragma unroll
-          for (int u = 0; u < U; u++)
-            fold<V, kFast>(rr[u], beta[u], lc[u], split ? p0 + u : j[u], lwu[u], j[u] < dim,
-                           acc);
-        } else {
+        for (uint32_t i = i0; i < i0 + CB && i < n && !skip; i += U) {
+          const uint32_t gm = marks >> (i - i0);
+          const uint32_t p0 = cb + i;
+          // the current row runs past the group: no row checks needed
+          const bool inside = row_end > p0 + min((uint32_t)U, n - i);
+          if (Bounded && inside && __ballot((gm & ((1u << U) - 1u)) != 0u) == 0ull) continue;
+          uint32_t j[U];
+          float lwu[U];
+          float rr[U][V], beta[U][V], lc[U][V];
 #pragma unroll
           for (int u = 0; u < U; u++) {
-            const uint32_t p = p0 + u;
-            fold<V, kFast>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u],
-                           j[u] < dim && !skip, acc);
-            if (i + u < n && p + 1 == row_end) {
-              flush_row<V>(M.output, r, samples, k0, split, acc);
-              if (p + 1 < b) next_row();
+            j[u] = readlane_u32(c, i + u);
+            lwu[u] = readlane_f32(lw, i + u);
+            const __amdgpu_buffer_rsrc_t rs = block_rsrc(
+                pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
+            const bool g = !Bounded || ((gm >> u) & 1u) != 0u;
+            load_b<V, kStream>(rs, g ? lo.r : kNoGather, rr[u]);
+            load_b<V, kStream>(rs, g ? lo.beta : kNoGather, beta[u]);
+            load_b<V, kStream>(rs, g ? lo.lc : kNoGather, lc[u]);
+          }
+          if (inside) {
+#pragma unroll
+            for (int u = 0; u < U; u++)
+              fold<V, kFast>(rr[u], beta[u], lc[u], split ? p0 + u : j[u], lwu[u],
+                             j[u] < dim && (!Bounded || ((gm >> u) & 1u) != 0u), acc);
+          } else {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+              const uint32_t p = p0 + u;
+              fold<V, kFast>(rr[u], beta[u], lc[u], split ? p : j[u], lwu[u],
+                             j[u] < dim && !skip && (!Bounded || ((gm >> u) & 1u) != 0u), acc);
+              if (i + u < n && p + 1 == row_end) {
+                flush_row<V>(M.output, r, samples, k0, split, acc);
+                if (p + 1 < b) next_row();
+              }
             }
           }
         }
@@ -1182,7 +1292,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
           if (piece) {
             const uint64_t col0 = (uint64_t)k * stripes.width;
             const uint32_t c0 = col0 < dim ? (uint32_t)col0 : dim;
-            if (codes != nullptr) {
+            if (codes != nullptr && samples % 8u == 0u) {
               // the row's keyed minima of the lane's 8 samples (quarter
               // layout); once the earlier pieces left one for every sample
               // of the tile, the bound can skip most exact evaluations
@@ -1210,9 +1320,15 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
         const uint32_t k0 = tile * (kWave * V) + lane * V;
         const uint32_t koff = (k0 < samples) ? k0 : 0u;
         const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
-        hash_item<V, WMH_GROUP_NNZ / V>(params, dim, sp, samples, k0, lo, M, a, b, row, ps, pe,
-                                         row == ~0u ? skip_nnz : 0u, fast_div != 0u,
-                                         items_first && step == 0);
+        const BoundCodes bc{codes, code_scale, code_t0};
+        if (codes != nullptr)
+          hash_item<V, WMH_GROUP_NNZ / V, true>(params, dim, sp, samples, k0, lo, M, a, b, row,
+                                                 ps, pe, row == ~0u ? skip_nnz : 0u,
+                                                 fast_div != 0u, items_first && step == 0, bc);
+        else
+          hash_item<V, WMH_GROUP_NNZ / V, false>(params, dim, sp, samples, k0, lo, M, a, b, row,
+                                                  ps, pe, row == ~0u ? skip_nnz : 0u,
+                                                  fast_div != 0u, items_first && step == 0, bc);
       }
     }
   }
@@ -1637,8 +1753,9 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
                              hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end,
                              CodeTable table) {
   const uint32_t sp = padded_samples(samples);
-  // the bounded pieces need 8 whole samples per lane
-  const uint8_t *codes = samples % 8u == 0u ? table.codes : nullptr;
+  // (the bounded stripe pieces also need 8 whole samples per lane: the
+  // kernel checks samples % 8)
+  const uint8_t *codes = table.codes;
   const int spl = tuning.spl;
   if (spl != 1 && spl != 2 && spl != 4) return hipErrorInvalidValue;
   if (tuning.seg <= 0 || tuning.seg % kWave != 0 || count == 0) return hipErrorInvalidValue;

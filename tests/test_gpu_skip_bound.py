@@ -1,11 +1,14 @@
-"""GPU parity of the skip bound (wmh_bound.h): column-stripe pieces whose row
-already holds a minimum for every sample of the tile skip the exact
-evaluation of (nonzero, sample) pairs whose lower bound is above it.  The
-hashes must be the same with the skip on and off and equal the oracle's:
-for sample counts that fill one or two 128-sample tiles or leave part of a
-tile empty, for a count that is not a multiple of 8 (the bound is not used),
-for tables outside the bound's range (not used), and for heavy-tailed,
-zero, infinite and unit weights inside long striped rows."""
+"""GPU parity of the skip bound (wmh_bound.h): the persistent kernel's row
+walks and its column-stripe pieces (once their row holds a minimum for every
+sample of the tile) skip the exact evaluation of (nonzero, sample) pairs
+whose lower bound is above the running minimum.  The hashes must be the same
+with the skip on and off and equal the oracle's: for sample counts that fill
+one or two 128-sample tiles or leave part of a tile empty, for a count that
+is not a multiple of 8 (the stripe pieces do not use the bound, the row walks
+do), for every lane width, for rows split across work items, for tables
+outside the bound's range (not used), and for heavy-tailed, zero, negative,
+NaN, subnormal, infinite and unit weights and columns >= dim inside long
+rows."""
 import ctypes
 
 import numpy as np
@@ -47,7 +50,7 @@ def test_skip_on_and_off_equal_the_oracle(libmhcuda, capi, gpu_available, sample
         params = libmhcuda.minhash_cuda_retrieve_vars(gen)
         assert capi.mhcuda_set_stripes(g, 1) == 0
         assert capi.mhcuda_set_tuning(g, 2, 1024, 0) == 0
-        assert skip_state(capi, g) == (1, int(samples % 8 == 0))
+        assert skip_state(capi, g) == (1, 1)
         on = libmhcuda.minhash_cuda_calc_batch(gen, mats, kernel=PERSISTENT)
         assert capi.mhcuda_set_skip(g, 0) == 0
         assert skip_state(capi, g) == (0, 0)
@@ -133,5 +136,74 @@ def test_extreme_weights_in_striped_rows(libmhcuda, capi, gpu_available):
             assert capi.mhcuda_set_tuning(g, 2, seg, 0) == 0
             got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
             assert np.array_equal(got, oracle.calc(*params, m.data, m.indices, m.indptr)), seg
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+
+
+def special_weights(m, rng, count):
+    """Puts zeros, negatives, NaN, subnormals, the largest floats and unit
+    weights at random positions of m, and one infinite weight early in its
+    first long row (that row's minimum is -inf from then on)."""
+    special = np.array([0.0, -2.0, np.nan, 1e-42, 3e38, 1.0], np.float32)
+    pick = rng.choice(m.nnz, count, replace=False)
+    m.data[pick] = special[rng.integers(0, len(special), count)]
+    lens = np.diff(m.indptr)
+    long_row = int(np.flatnonzero(lens >= 400)[0])
+    m.data[m.indptr[long_row] + 37] = np.inf
+    return m
+
+
+@pytest.mark.parametrize("samples,spl", [(2, 1), (64, 1), (100, 2), (128, 2), (128, 4),
+                                         (200, 4), (256, 4), (256, 2)])
+def test_row_walk_bound_equals_the_oracle(libmhcuda, capi, gpu_available, samples, spl):
+    """Unstriped rows of 1-5,000 nonzeros (most of them hashed by the row
+    walk's bounded blocks), some columns >= dim, special weights; items of
+    64, 512 and 4096 nonzeros (many rows split across items); bound on and
+    off."""
+    print("samples", samples, "spl", spl, flush=True)
+    dim = 1 << 15
+    rng = np.random.default_rng(samples * 10 + spl)
+    nnz = np.concatenate([rng.integers(1, 64, 40), rng.integers(64, 1500, 20), [3000, 5000]])
+    rng.shuffle(nnz)
+    m = workloads.batched_csr(nnz.astype(np.int64), dim=dim + 9, sigma=2.0, seed=samples + spl)
+    m = special_weights(m, rng, 500)
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=samples + 1, devices=1)
+    g = ctypes.c_void_p(gen)
+    try:
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+        ref = oracle.calc(*params, m.data, m.indices, m.indptr)
+        assert capi.mhcuda_set_stripes(g, 0) == 0
+        assert skip_state(capi, g) == (1, 1)
+        for seg in (64, 512, 4096):
+            assert capi.mhcuda_set_tuning(g, spl, seg, 0) == 0
+            got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+            assert np.array_equal(got, ref), seg
+        assert capi.mhcuda_set_skip(g, 0) == 0
+        off = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+        assert np.array_equal(off, ref)
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+
+
+def test_row_walk_bound_with_stripes_and_short_rows(libmhcuda, capi, gpu_available):
+    """Striped long rows beside bounded row walks of the other rows, at 4
+    samples per lane (stripe pieces through the row walk) and at 2 (paired
+    and bounded stripe pieces), 256 samples."""
+    dim, samples = 1 << 15, 256
+    rng = np.random.default_rng(5)
+    nnz = np.concatenate([rng.integers(1, 900, 60), rng.integers(1100, 6000, 8)])
+    rng.shuffle(nnz)
+    m = workloads.batched_csr(nnz.astype(np.int64), dim=dim, sigma=1.5, seed=55)
+    m = special_weights(m, rng, 300)
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=6, devices=1)
+    g = ctypes.c_void_p(gen)
+    try:
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+        ref = oracle.calc(*params, m.data, m.indices, m.indptr)
+        assert capi.mhcuda_set_stripes(g, 1) == 0
+        for spl in (4, 2):
+            assert capi.mhcuda_set_tuning(g, spl, 1024, 0) == 0
+            got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+            assert np.array_equal(got, ref), spl
     finally:
         libmhcuda.minhash_cuda_fini(gen)
