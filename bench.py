@@ -25,7 +25,10 @@ Workloads (parameters from the device Philox generator):
            weights, 128 samples, 18.4 GB of parameters resident: one resident
            65,536-row chunk, plus all 1M rows end to end from pinned host memory.
   serve    continuous-batching serving loop: Poisson arrivals at a QPS sweep,
-           requests of 1-4096 rows, dim 2^22, 256 samples, 12.9 GB resident.
+           requests of 1-4096 rows, dim 2^22, 256 samples, 12.9 GB resident;
+           plus the hashing kernel on the largest launch the loop makes (the
+           256 pool requests coalesced, device-resident) and its roofline.
+  serve_kernel  that kernel measurement alone (for profiling).
   c1       the reference's CPU-runnable case (6400 x 130, test.py recipe).
 """
 import argparse
@@ -57,7 +60,7 @@ def parse(argv=None):
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--workload", default="batched",
-                   choices=["batched", "plain", "serve", "large", "c1", "copy"],
+                   choices=["batched", "plain", "serve", "serve_kernel", "large", "c1", "copy"],
                    help="copy: a plain 4 GiB device-to-device copy (calibrates counters)")
     p.add_argument("--qps", default=None,
                    help="serve: comma-separated arrival rates (requests/s)")
@@ -398,16 +401,98 @@ def serve_requests(n, dim, seed):
     return reqs
 
 
-def serve_sweep(args, torch, qps_list, seconds):
+SERVE_DIM, SERVE_SAMPLES, SERVE_POOL = 1 << 22, 256, 256
+
+
+def serve_engine(args):
+    """The serving generator (dim 2^22, 256 samples, seed 1) with the command
+    line's tuning, if any."""
+    import libMHCUDA
+    gen = libMHCUDA.minhash_cuda_init(SERVE_DIM, SERVE_SAMPLES, seed=1, devices=1)
+    if explicit_tuning(args):
+        from minhashcuda_amd import capi as capi_mod
+        capi = capi_mod.load(os.path.join(PKG, "libMHCUDA.so"))
+        g = ctypes.c_void_p(gen)
+        capi_mod.check(capi.mhcuda_set_tuning(g, args.spl, args.seg, args.wpc), "tuning")
+        if args.stripe_mib >= 0:
+            capi_mod.check(capi.mhcuda_set_stripes(g, args.stripe_mib), "stripes")
+    return gen
+
+
+def serve_kernel(args, torch, gen, pool, steps=5):
+    """The hashing pass of the largest launch the serving loop makes: all
+    requests of the pool coalesced (the server's max_batch_requests),
+    device-resident, one persistent launch with the generator's tuning.
+    Returns its rate, kernel time and roofline."""
+    import numpy as np
+    from minhashcuda_amd import capi as cm, workloads
+    capi = cm.load(os.path.join(PKG, "libMHCUDA.so"))
+    g = ctypes.c_void_p(gen)
+    cm.check(capi.mhcuda_set_timing(g, 1), "timing")
+    tens = []
+    for m in pool:
+        tens.append((torch.from_numpy(m.data.astype(np.float32)).cuda(),
+                     torch.from_numpy(m.indices.astype(np.int32)).cuda(),
+                     torch.from_numpy(m.indptr.astype(np.int64).astype(np.int32)).cuda(),
+                     m.shape[0]))
+    outs = [torch.empty((t[3], SERVE_SAMPLES, 2), dtype=torch.int32, device="cuda")
+            for t in tens]
+    descs = (cm.Matrix * len(tens))()
+    for i, ((w, c, r, n), o) in enumerate(zip(tens, outs)):
+        descs[i] = cm.Matrix(w.data_ptr(), c.data_ptr(), r.data_ptr(), n, o.data_ptr())
+    stream = torch.cuda.current_stream()
+
+    def run():
+        cm.check(capi.mhcuda_calc_batch_device(g, len(tens), descs, cm.KERNEL_PERSISTENT,
+                                               ctypes.c_void_p(stream.cuda_stream)), "calc")
+
+    class _Timer:
+        def kernel_ms(self):
+            ms = ctypes.c_float()
+            cm.check(capi.mhcuda_kernel_time(g, ctypes.byref(ms)), "kernel_time")
+            return ms.value
+
+    torch.cuda.synchronize()
+    wall, kms = timed_steps(torch, run, _Timer(), steps, 2)
+    cm.check(capi.mhcuda_set_timing(g, 0), "timing")
+    nnz = sum(m.nnz for m in pool)
+    rows = sum(m.shape[0] for m in pool)
+    avg_ms = sum(kms) / len(kms)
+    mib, k = ctypes.c_int(), ctypes.c_uint32()
+    capi.mhcuda_get_stripes(g, ctypes.byref(mib), ctypes.byref(k), None)
+    alg = workloads.algorithmic_bytes(nnz, rows, SERVE_SAMPLES)
+    res = {"requests": len(pool), "rows": rows, "nnz": nnz, "steps": steps,
+           "value": nnz * SERVE_SAMPLES * steps / wall, "ms_per_step": wall * 1e3 / steps,
+           "avg_kernel_ms": avg_ms,
+           "roofline": roofline("serve", "wmh_persistent_kernel", alg, avg_ms, k.value)}
+    del tens, outs, descs
+    torch.cuda.empty_cache()
+    return res
+
+
+def serve_kernel_bench(args, torch):
+    import libMHCUDA
+    log("serve: generating request pool")
+    pool = serve_requests(SERVE_POOL, SERVE_DIM, seed=5)
+    gen = serve_engine(args)
+    res = serve_kernel(args, torch, gen, pool, steps=args.steps or 5)
+    libMHCUDA.minhash_cuda_fini(gen)
+    print(json.dumps(dict({"metric": METRIC, "workload": "serve_kernel",
+                           "unit": "nnz*samples/s", "n_gpus": 1, "dtype": "f32",
+                           "data": "synthetic"}, **res)), flush=True)
+
+
+def serve_sweep(args, torch, qps_list, seconds, with_kernel=False):
     """Continuous-batching serving loop under Poisson arrivals at each rate:
-    p50/p99 latency (submit -> hashes in host memory) and throughput."""
+    p50/p99 latency (submit -> hashes in host memory) and throughput; with
+    with_kernel, also serve_kernel on the same pool and generator."""
     import threading
     import numpy as np
     import libMHCUDA
     from minhashcuda_amd import serving
-    dim, samples = 1 << 22, 256
+    dim, samples = SERVE_DIM, SERVE_SAMPLES
     log("serve: generating request pool")
-    pool = serve_requests(256, dim, seed=5)
+    pool = serve_requests(SERVE_POOL, dim, seed=5)
     pinned_pool = []
     if args.pinned:
         for m in pool:
@@ -422,15 +507,13 @@ def serve_sweep(args, torch, qps_list, seconds):
             ot = torch.empty((m.shape[0], samples, 2), dtype=torch.int32, pin_memory=True)
             arrs.append((ot, ot.numpy().view(np.uint32)))
             pinned_pool.append(arrs)
-    gen = libMHCUDA.minhash_cuda_init(dim, samples, seed=1, devices=1)
-    if explicit_tuning(args):
-        from minhashcuda_amd import capi as capi_mod
-        capi = capi_mod.load(os.path.join(PKG, "libMHCUDA.so"))
-        g = ctypes.c_void_p(gen)
-        capi_mod.check(capi.mhcuda_set_tuning(g, args.spl, args.seg, args.wpc), "tuning")
-        if args.stripe_mib >= 0:
-            capi_mod.check(capi.mhcuda_set_stripes(g, args.stripe_mib), "stripes")
-    server = serving.Server(gen, memory_budget=4 << 30, slots=args.slots, max_batch_requests=256)
+    gen = serve_engine(args)
+    kernel = None
+    if with_kernel:
+        log("serve: hashing kernel on the %d pool requests coalesced" % len(pool))
+        kernel = serve_kernel(args, torch, gen, pool)
+    server = serving.Server(gen, memory_budget=4 << 30, slots=args.slots,
+                            max_batch_requests=SERVE_POOL)
     # warm-up: every buffer reaches its working size
     for t in [server.submit(m) for m in pool[:64]]:
         server.wait(t)
@@ -490,7 +573,7 @@ def serve_sweep(args, torch, qps_list, seconds):
               "nnz/row lognormal(median 200, sigma 1) clipped to 50000, lognormal(0,1) weights, "
               "256 samples, parameters resident (12.9 GB), %d slots, 4 GiB budget, request "
               "arrays in %s host memory" % (args.slots, "pinned" if args.pinned else "pageable"))
-    return results, config, len(pool)
+    return results, config, len(pool), kernel
 
 
 def serve_summary(results):
@@ -510,16 +593,20 @@ def serve_summary(results):
     }
 
 
+SERVE_QPS = "2000,4000,6000,8000,10000,12000,14000,16000"
+
+
 def serve_bench(args, torch):
-    qps = [float(q) for q in (args.qps or "1000,3000,6000,9000,12000").split(",")]
-    results, config, pool = serve_sweep(args, torch, qps, args.serve_seconds or 3.0)
+    qps = [float(q) for q in (args.qps or SERVE_QPS).split(",")]
+    results, config, pool, kernel = serve_sweep(args, torch, qps, args.serve_seconds or 3.0,
+                                                with_kernel=True)
     best = max(results, key=lambda r: r["nnz_samples_per_sec"])
     print(json.dumps({
         "metric": METRIC, "workload": "serve", "unit": "nnz*samples/s",
         "value": best["nnz_samples_per_sec"], "higher_is_better": True, "n_gpus": 1,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": config, "pool_requests": pool},
-        "summary": serve_summary(results), "sweep": results}), flush=True)
+        "summary": serve_summary(results), "kernel": kernel, "sweep": results}), flush=True)
 
 
 # ---------------------------------------------------------------------------
@@ -725,6 +812,9 @@ def main(argv=None):
     if args.workload == "serve":
         serve_bench(args, torch)
         return
+    if args.workload == "serve_kernel":
+        serve_kernel_bench(args, torch)
+        return
     if args.workload == "large":
         large_bench(args, torch)
         return
@@ -852,10 +942,11 @@ def main(argv=None):
         result["large"] = large_run(args, torch, 3, 1, False)
     if args.workload == "batched" and not args.no_serve:
         log("serving sweep")
-        qps = [float(q) for q in (args.qps or "2000,4000,6000,8000,10000,12000").split(",")]
-        sweep, config, pool = serve_sweep(args, torch, qps, args.serve_seconds or 2.0)
+        qps = [float(q) for q in (args.qps or SERVE_QPS).split(",")]
+        sweep, config, pool, kernel = serve_sweep(args, torch, qps, args.serve_seconds or 2.0,
+                                                  with_kernel=True)
         result["serve"] = dict(serve_summary(sweep), config=config, pool_requests=pool,
-                               sweep=sweep)
+                               kernel=kernel, sweep=sweep)
     print(json.dumps(result), flush=True)
 
 
