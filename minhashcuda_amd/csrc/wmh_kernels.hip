@@ -564,6 +564,7 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
 #endif
 constexpr uint32_t kCodeGroups = WMH_CODE_GROUPS;
 
+
 // Buffer offset past every column block (block sizes are far below 2^31):
 // an unmarked lane's gather returns zero and reads no memory.
 constexpr uint32_t kNoGather = 0x80000000u;
@@ -634,6 +635,14 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   // positions whose codes are loaded together; codes loaded a block ahead
   constexpr uint32_t CB = kCodeGroups * (uint32_t)U;
   constexpr bool kPrefetch = V == 4;
+  // A block uses the codes once it starts kBoundAfter positions into its
+  // row's part of [a, b): closer to the row's start its minima rarely skip
+  // enough to repay the codes' round trip.  At 1 or 2 samples per lane a
+  // 128-byte line of a parameter array serves 32 or 16 lanes, so the bound
+  // saves HBM traffic only late in a row (measured on 100,000 rows of
+  // Poisson(100) nonzeros: +6.5% at 64 against 0); at 4 samples per lane 16
+  // (serving rows: 64 cost 1%, 128 cost 6%).
+  constexpr uint32_t kBoundAfter = V == 4 ? 16u : 64u;
   static_assert(kWave % CB == 0, "blocks tile a chunk");
   const uint32_t lane = lane_id();
   const char *pbase = reinterpret_cast<const char *>(params);
@@ -657,6 +666,8 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
   // holds the whole row (the fix-up pass reads every striped row as keys)
   bool split = in_row || row_start < a || row_end > b;
   bool skip = skip_nnz != 0u && row_end - row_start >= skip_nnz;
+  // the current row's first position in [a, b): its minima start there
+  uint32_t row_first = row_start > a ? row_start : a;
   Argmin<V> acc;
   acc.init();
 
@@ -673,6 +684,7 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
       }
       row_end = readlane_u32(rp, rp_idx);
     } while (row_end == row_start);
+    row_first = row_start;
     split = row_end > b;
     skip = skip_nnz != 0u && row_end - row_start >= skip_nnz;
     acc.init();
@@ -715,22 +727,33 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
           x[u] = load_codes<V>(rs, koff);
         }
       };
+      auto codes_at = [&](uint32_t i0) { return cb + i0 - row_first >= kBoundAfter; };
       uint32_t xn[kPrefetch ? CB : 1];
-      if constexpr (Bounded && kPrefetch) load_block(0, xn);
+      bool have_xn = false;
+      if constexpr (Bounded && kPrefetch) {
+        have_xn = codes_at(0);
+        if (have_xn) load_block(0, xn);
+      }
       for (uint32_t i0 = 0; i0 < n && !skip; i0 += CB) {
         // bit u: this lane folds position cb + i0 + u
         uint32_t marks = 0xffffffffu;
+        bool use_codes = false;
+        uint32_t x[CB];
         if constexpr (Bounded) {
-          uint32_t tp[(V + 1) / 2];
-          lane_thresholds<V>(acc.best, bc.scale, bc.t0, tp);
-          uint32_t x[CB];
           if constexpr (kPrefetch) {
+            use_codes = have_xn && codes_at(i0);
 #pragma unroll
             for (int u = 0; u < (int)CB; u++) x[u] = xn[u];
-            if (i0 + CB < n) load_block(i0 + CB, xn);
+            have_xn = i0 + CB < n && codes_at(i0 + CB);
+            if (have_xn) load_block(i0 + CB, xn);
           } else {
-            load_block(i0, x);
+            use_codes = codes_at(i0);
+            if (use_codes) load_block(i0, x);
           }
+        }
+        if (Bounded && use_codes) {
+          uint32_t tp[(V + 1) / 2];
+          lane_thresholds<V>(acc.best, bc.scale, bc.t0, tp);
           marks = 0u;
 #pragma unroll
           for (int u = 0; u < (int)CB; u++) {

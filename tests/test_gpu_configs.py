@@ -1,7 +1,15 @@
-"""GPU parity at the full shapes of the large single-GPU share and of the
-serving workload, where the parameter tables are too large for a host copy:
-the oracle recomputes only the table columns the checked rows touch
-(oracle.calc_touched).
+"""GPU parity at the full shapes of the uneven batch, the large single-GPU
+share and the serving workload.  Where the parameter tables are too large for
+a host copy the oracle recomputes only the table columns the checked rows
+touch (oracle.calc_touched).
+
+* uneven batch: 64 matrices, dim 2^20, rows per matrix log-uniform in
+  [1k, 200k], nnz/row Zipf(1.2) clipped to [1, 20000], lognormal(0, 1.5)
+  weights, 128 samples (2.53M rows, 7.8e9 nonzeros), generated in device
+  memory: one persistent launch at the tuning the autotuner picks for it
+  equals the 64 plain launches in every hash, and sampled rows of three
+  matrices (the longest, striped and skip-bounded, the first and last, and
+  random ones) equal the oracle.
 
 * 1,000,000 x 12,000,000 share: rows of 16,800 sorted uniform columns,
   uniform (0, 100] weights, 128 samples, the 18.4 GB generated table, 1,024
@@ -41,6 +49,67 @@ def host_rows(w, c, r, rows, dim):
         indptr.append(indptr[-1] + (e - s))
     return sp.csr_matrix((np.concatenate(parts_w), np.concatenate(parts_c),
                           np.asarray(indptr, np.int64)), shape=(len(rows), dim))
+
+
+def device_rows(w, c, r, rows, dim):
+    """scipy CSR of the selected rows of a matrix held in device tensors."""
+    ip = r.cpu().numpy().view(np.uint32).astype(np.int64)
+    parts_w, parts_c, indptr = [], [], [0]
+    for i in rows:
+        s, e = int(ip[i]), int(ip[i + 1])
+        parts_w.append(w[s:e].cpu().numpy())
+        parts_c.append(c[s:e].cpu().numpy().view(np.uint32))
+        indptr.append(indptr[-1] + (e - s))
+    return sp.csr_matrix((np.concatenate(parts_w), np.concatenate(parts_c),
+                          np.asarray(indptr, np.int64)), shape=(len(rows), dim))
+
+
+def test_uneven_batch_full_shape(libmhcuda, capi, gpu_available):
+    dim, samples = 1 << 20, 128
+    layout = workloads.batched_layout(count=64, seed=1)
+    print("uneven batch: generating %d rows, %d nonzeros" % (
+        sum(len(n) for n in layout), sum(int(n.sum()) for n in layout)), flush=True)
+    tensors = [workloads.batched_csr_torch(n, dim, 1.5, seed=100 + i)
+               for i, n in enumerate(layout)]
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=1, devices=1)
+    g = ctypes.c_void_p(gen)
+    try:
+        assert capi.mhcuda_set_tuning(g, 2, 4096, 16) == 0
+        assert capi.mhcuda_set_stripes(g, 32) == 0
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        outs = {}
+        for kernel in (capi_mod.KERNEL_PERSISTENT, capi_mod.KERNEL_PLAIN):
+            o = [torch.zeros((len(n), samples, 2), dtype=torch.int32, device="cuda")
+                 for n in layout]
+            desc = (capi_mod.Matrix * len(layout))()
+            for i, ((w, c, r), oi) in enumerate(zip(tensors, o)):
+                desc[i] = capi_mod.Matrix(w.data_ptr(), c.data_ptr(), r.data_ptr(),
+                                          len(layout[i]), oi.data_ptr())
+            print("uneven batch: kernel %d" % kernel, flush=True)
+            assert capi.mhcuda_calc_batch_device(g, len(layout), desc, kernel, stream) == 0
+            torch.cuda.synchronize()
+            outs[kernel] = o
+        pers, plain = outs[capi_mod.KERNEL_PERSISTENT], outs[capi_mod.KERNEL_PLAIN]
+        differ = [i for i, (a, b) in enumerate(zip(pers, plain)) if not torch.equal(a, b)]
+        assert differ == [], "matrices whose hashes differ between the kernels: %s" % differ
+        del outs, plain
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+        rng = np.random.default_rng(3)
+        for m in (0, 31, 63):
+            nnz = layout[m]
+            longest = np.argsort(nnz)[-5:]
+            pick = np.unique(np.concatenate([[0, 1, 2, len(nnz) - 1], longest,
+                                             rng.integers(0, len(nnz), 40)]))
+            sub = device_rows(*tensors[m], pick, dim)
+            print("uneven batch: oracle on matrix %d, %d rows (%d nnz, longest %d)" % (
+                m, len(pick), sub.nnz, int(nnz.max())), flush=True)
+            ref = oracle.calc_omp(*params, sub.data, sub.indices, sub.indptr, 16)[0]
+            got = pers[m][torch.as_tensor(pick, device="cuda")].cpu().numpy().view(np.uint32)
+            assert np.array_equal(got, ref), m
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
+        del tensors
+        torch.cuda.empty_cache()
 
 
 def test_large_share_full_shape(libmhcuda, capi, gpu_available):
