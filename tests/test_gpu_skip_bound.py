@@ -153,8 +153,9 @@ def special_weights(m, rng, count):
     return m
 
 
-@pytest.mark.parametrize("samples,spl", [(2, 1), (64, 1), (100, 2), (128, 2), (128, 4),
-                                         (200, 4), (256, 4), (256, 2)])
+@pytest.mark.parametrize("samples,spl", [(2, 1), (34, 1), (64, 1), (100, 2), (128, 2),
+                                         (130, 2), (128, 4), (200, 4), (202, 4), (256, 4),
+                                         (256, 2)])
 def test_row_walk_bound_equals_the_oracle(libmhcuda, capi, gpu_available, samples, spl):
     """Unstriped rows of 1-5,000 nonzeros (most of them hashed by the row
     walk's bounded blocks), some columns >= dim, special weights; items of
