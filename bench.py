@@ -563,10 +563,14 @@ def serve_sweep(args, torch, qps_list, seconds, with_kernel=False):
                         "nnz_samples_per_sec": st["nnz_samples_per_sec"],
                         "mean_requests_per_batch": st["mean_requests_per_batch"],
                         "launches": st["batches"], "mean_rows_per_request": rows / n,
-                        "mean_nnz_per_request": nnz / n})
-        log("qps %.0f: p50 %.2f ms p99 %.2f ms, %.3g nnz*S/s, %.1f req/launch" % (
-            qps, st["p50_ms"], st["p99_ms"], st["nnz_samples_per_sec"],
-            st["mean_requests_per_batch"]))
+                        "mean_nnz_per_request": nnz / n,
+                        "phases_ms": {k: st[k] for k in ("queue_ms", "stage_in_ms",
+                                                         "device_ms", "stage_out_ms")}})
+        log("qps %.0f: p50 %.2f ms p99 %.2f ms, %.3g nnz*S/s, %.1f req/launch; queue %.2f, "
+            "per launch: stage in %.2f, device %.2f, stage out %.2f ms" % (
+                qps, st["p50_ms"], st["p99_ms"], st["nnz_samples_per_sec"],
+                st["mean_requests_per_batch"], st["queue_ms"], st["stage_in_ms"],
+                st["device_ms"], st["stage_out_ms"]))
     server.stop()
     libMHCUDA.minhash_cuda_fini(gen)
     config = ("serving: Poisson arrivals, requests of 1-4096 rows (log-uniform), dim 2^22, "

@@ -325,6 +325,12 @@ typedef struct {
   double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> hashes in host memory */
   double window_s;              /* first submit -> last completion */
   double nnz_samples_per_sec;   /* completed nonzeros * samples / window_s */
+  /* where the time goes (means): per request, submit -> admitted into a slot;
+     per launch, the host side of the uploads (staging pageable arrays through
+     pinned blocks, enqueueing the copies), then the device side (the hashing
+     launch and the downloads, until they complete), then copying hashes out
+     of the staging blocks into pageable outputs */
+  double queue_ms, stage_in_ms, device_ms, stage_out_ms;
 } MHCUDAServeStats;
 
 /* Starts the scheduler, slot workers and their device pool (memory_budget

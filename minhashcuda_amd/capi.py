@@ -72,7 +72,9 @@ class ServeStats(ctypes.Structure):
                 ("batches", ctypes.c_uint64), ("mean_requests_per_batch", ctypes.c_double),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
-                ("window_s", ctypes.c_double), ("nnz_samples_per_sec", ctypes.c_double)]
+                ("window_s", ctypes.c_double), ("nnz_samples_per_sec", ctypes.c_double),
+                ("queue_ms", ctypes.c_double), ("stage_in_ms", ctypes.c_double),
+                ("device_ms", ctypes.c_double), ("stage_out_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

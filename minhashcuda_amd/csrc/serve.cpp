@@ -45,7 +45,7 @@ struct Request {
   uint64_t nnz = 0;
   uint64_t bytes = 0;   // device bytes while in flight
   uint64_t ticket = 0;
-  Clock::time_point submitted;
+  Clock::time_point submitted, admitted;
   MHCUDAResult status = mhcudaSuccess;
   bool done = false;
   double latency_ms = 0;
@@ -182,6 +182,11 @@ struct StageRing {
 // range of the server's device pool), enqueues their hashing and downloads,
 // waits for them and hands the hashes back, so host-side copies of different
 // slots run in parallel while their launches overlap on the device.
+// Host-clock phases of one launch (Clock::now() at each boundary).
+struct Phases {
+  Clock::time_point start, uploaded, finished, drained;
+};
+
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t finished = nullptr;
@@ -189,6 +194,7 @@ struct Slot {
   StageRing ring;
   uint64_t pool_off = 0, pool_bytes = 0;  // this group's range of the pool
   std::vector<Request *> reqs;
+  Phases phases;
   bool busy = false;      // requests assigned and not yet retired
   std::thread worker;
   std::condition_variable cv;
@@ -214,6 +220,8 @@ struct MHCUDAServer_ {
   // statistics since the last reset
   std::vector<double> latencies;
   uint64_t rejected = 0, batches = 0, batched_requests = 0, units = 0;
+  uint64_t timed_groups = 0;  // successful launches behind the phase sums
+  double queue_sum = 0, stage_in_sum = 0, device_sum = 0, stage_out_sum = 0;
   Clock::time_point first_submit, last_done;
   bool have_window = false;
 
@@ -264,6 +272,7 @@ struct MHCUDAServer_ {
   // Uploads, hashes and downloads the requests of `slot`; returns when the
   // hashes are in the requests' output arrays.
   MHCUDAResult run(Slot *slot) {
+    slot->phases.start = Clock::now();
     char *base = (char *)pool.ptr + slot->pool_off;
     const uint64_t S8 = (uint64_t)gen->samples * 8;
     uint64_t off = 0;
@@ -292,6 +301,7 @@ struct MHCUDAServer_ {
                                       (const uint32_t *)dr, (uint32_t *)dh, r->length, first,
                                       0u, 0u, 0u, 0u});
     }
+    slot->phases.uploaded = Clock::now();
     // (untimed: the generator's timing state belongs to the calls that hold
     // its lock; serving keeps its own latency statistics)
     MHCUDAResult res = wmh::hash_device_batch(gen, descs, mhcudaKernelPersistent, slot->stream,
@@ -304,8 +314,11 @@ struct MHCUDAServer_ {
         return mhcudaMemoryCopyError;
     }
     if (hipEventRecord(slot->finished, slot->stream) != hipSuccess ||
-        hipEventSynchronize(slot->finished) != hipSuccess || !slot->ring.drain(slot->stream))
+        hipEventSynchronize(slot->finished) != hipSuccess)
       return mhcudaRuntimeError;
+    slot->phases.finished = Clock::now();
+    if (!slot->ring.drain(slot->stream)) return mhcudaRuntimeError;
+    slot->phases.drained = Clock::now();
     return mhcudaSuccess;
   }
 
@@ -326,14 +339,25 @@ struct MHCUDAServer_ {
       }
       const Clock::time_point now = Clock::now();
       lock.lock();
+      auto ms = [](Clock::time_point a, Clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+      };
       for (Request *r : slot->reqs) {
         r->status = st;
         r->done = true;
-        r->latency_ms = std::chrono::duration<double, std::milli>(now - r->submitted).count();
+        r->latency_ms = ms(r->submitted, now);
         if (st == mhcudaSuccess) {
           latencies.push_back(r->latency_ms);
           units += r->nnz * gen->samples;
+          queue_sum += ms(r->submitted, r->admitted);
         }
+      }
+      if (st == mhcudaSuccess) {
+        const Phases &ph = slot->phases;
+        timed_groups++;
+        stage_in_sum += ms(ph.start, ph.uploaded);
+        device_sum += ms(ph.uploaded, ph.finished);
+        stage_out_sum += ms(ph.finished, ph.drained);
       }
       if (now > last_done) last_done = now;
       ranges.give(slot->pool_off, slot->pool_bytes);
@@ -367,9 +391,11 @@ struct MHCUDAServer_ {
       // admission: FIFO while the largest free range and the request cap allow
       const uint64_t room = ranges.largest();
       uint64_t used = 0;
+      const Clock::time_point now = Clock::now();
       while (!pending.empty() && slot->reqs.size() < cfg.max_batch_requests &&
              used + pending.front()->bytes <= room) {
         used += pending.front()->bytes;
+        pending.front()->admitted = now;
         slot->reqs.push_back(pending.front());
         pending.pop_front();
       }
@@ -521,11 +547,18 @@ MHCUDAResult mhcuda_serve_stats(MHCUDAServer *srv, MHCUDAServeStats *out, int re
           : 0;
   s.window_s = window;
   s.nnz_samples_per_sec = window > 0 ? (double)srv->units / window : 0;
+  s.queue_ms = s.completed ? srv->queue_sum / s.completed : 0;
+  const double groups = (double)srv->timed_groups;
+  s.stage_in_ms = groups > 0 ? srv->stage_in_sum / groups : 0;
+  s.device_ms = groups > 0 ? srv->device_sum / groups : 0;
+  s.stage_out_ms = groups > 0 ? srv->stage_out_sum / groups : 0;
   *out = s;
   if (reset) {
     srv->latencies.clear();
     srv->rejected = srv->batches = srv->batched_requests = srv->units = 0;
     srv->have_window = false;
+    srv->timed_groups = 0;
+    srv->queue_sum = srv->stage_in_sum = srv->device_sum = srv->stage_out_sum = 0;
   }
   return mhcudaSuccess;
 }

@@ -56,6 +56,12 @@ def test_concurrent_requests_equal_alone(gen_params, slots, budget):
     server.stop()
     assert stats["completed"] == 60 and stats["rejected"] == 0
     assert stats["p50_ms"] <= stats["p99_ms"] <= stats["max_ms"]
+    # phase means: a request's queueing is part of its latency, and one
+    # launch's phases add up to at most the longest latency
+    assert 0 <= stats["queue_ms"] <= stats["mean_ms"]
+    phases = stats["stage_in_ms"] + stats["device_ms"] + stats["stage_out_ms"]
+    assert min(stats["stage_in_ms"], stats["device_ms"], stats["stage_out_ms"]) >= 0
+    assert 0 < phases <= stats["max_ms"]
     for i, m in enumerate(reqs):
         ref = oracle.calc_omp(*params, m.data, m.indices, m.indptr, 16)[0]
         assert np.array_equal(results[i], ref), "request %d" % i
