@@ -556,7 +556,7 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
 // per SIMD; two groups need 141 at 4 samples per lane).  At 4 samples per
 // lane the codes of the next block are loaded while a block's parameters are
 // gathered, so a block waits for one round trip instead of two (+7% on a
-// serving batch at 256 samples, although its 130 VGPRs leave 3 waves per
+// serving batch at 256 samples, although its 134 VGPRs leave 3 waves per
 // SIMD; at 2 samples per lane the same would cost the stripe pieces, which
 // share that kernel, their fourth wave).
 #ifndef WMH_CODE_GROUPS

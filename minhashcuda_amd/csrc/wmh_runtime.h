@@ -198,7 +198,9 @@ struct MinhashCudaGenerator_ {
   uint64_t piece_rows = 1ull << 20;
   // 12 waves per CU by default: 3 per SIMD leave room in the register file
   // for the runtime's copy kernels of overlapped uploads and downloads (the
-  // hashing kernels use up to 128 VGPRs, so 4 waves per SIMD fill it)
+  // 1- and 2-samples-per-lane hashing kernels use up to 128 VGPRs, so 4 waves
+  // per SIMD fill it; the 4-samples-per-lane one uses 134, so 3 waves per
+  // SIMD are its full occupancy and still leave 104 registers free)
   wmh::Tuning tuning{2, 2048, 12, wmh::kDefaultStripeMib};
   bool seg_auto = true;  // let small batches use smaller work items
   wmh::GraphCache graph;  // hipGraph replay of device-pointer calls (opt-in)
