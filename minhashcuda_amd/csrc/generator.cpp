@@ -648,16 +648,49 @@ namespace {
 
 // The pipelined host-memory path of mhcuda_calc_batch (called with the
 // generator's lock held and its device current).
+// A large host-memory call is cut into about kPipelinePieces pieces (of at
+// least kMinPieceNnz nonzeros) so that the uploads, the hashing and the
+// downloads of consecutive pieces overlap.
+constexpr uint64_t kPipelinePieces = 4;
+constexpr uint64_t kMinPieceNnz = 1ull << 21;
+// Pieces whose inputs are pageable are staged through a pinned buffer per
+// buffer set: they hold at most kStagedPieceNnz nonzeros (256 MiB of weights
+// and columns).
+constexpr uint64_t kStagedPieceNnz = 1ull << 25;
+
 MHCUDAResult calc_batch_locked(MinhashCudaGenerator *gen, uint32_t count,
-                               const MHCUDAMatrix *mats, int kernel, uint64_t max_row) {
+                               const MHCUDAMatrix *mats, int kernel, uint64_t max_row,
+                               uint64_t total_nnz, uint64_t total_rows) {
   int verbosity = gen->verbosity;
   // Pieces: consecutive row ranges holding at most `budget` nonzeros and
   // `row_budget` rows (a row is never cut).  Piece p uses device buffer set
   // p % 2, so the upload of piece p + 1 overlaps the hashing of piece p and
   // the download of the hashes runs on a third stream.  The grow-only buffers
   // play the role of the reference's fill_buffers (minhashcuda.cc:385-455).
-  const uint64_t budget = std::max<uint64_t>(gen->piece_nnz, max_row);
-  const uint64_t row_budget = std::max<uint64_t>(1, gen->piece_rows);
+  const uint64_t split_nnz =
+      std::max(kMinPieceNnz, (total_nnz + kPipelinePieces - 1) / kPipelinePieces);
+  const uint64_t split_rows = std::max<uint64_t>(
+      1, split_nnz >= total_nnz ? total_rows
+                                : (total_rows + kPipelinePieces - 1) / kPipelinePieces);
+  bool pageable = false;
+  for (uint32_t m = 0; m < count && !pageable; m++) {
+    const MHCUDAMatrix &x = mats[m];
+    if (x.length == 0) continue;
+    pageable = !wmh::is_pinned(x.weights + x.rows[0]) || !wmh::is_pinned(x.cols + x.rows[0]) ||
+               !wmh::is_pinned(x.rows);
+  }
+  uint64_t budget = std::min(gen->piece_nnz, split_nnz);
+  if (pageable) budget = std::min(budget, kStagedPieceNnz);
+  budget = std::max<uint64_t>(budget, max_row);
+  const uint64_t row_budget = std::max<uint64_t>(1, std::min(gen->piece_rows, split_rows));
+  if (!gen->copier) {
+    try {
+      gen->copier.reset(new wmh::HostCopier());
+    } catch (...) {
+      return mhcudaRuntimeError;  // no threads
+    }
+  }
+  wmh::HostCopier &copier = *gen->copier;
   struct Span {
     uint32_t m, r0, r1;
   };
@@ -695,7 +728,7 @@ MHCUDAResult calc_batch_locked(MinhashCudaGenerator *gen, uint32_t count,
     wmh::PieceBuffers &buf = gen->pieces[b];
     // the set is free once its previous download has completed
     HIPCH(hipEventSynchronize(buf.done), mhcudaRuntimeError);
-    buf.settle();
+    buf.settle(copier);
     uint64_t nnz = 0, nrows = 0;
     for (const Span &sp : pieces[p]) {
       nnz += mats[sp.m].rows[sp.r1] - mats[sp.m].rows[sp.r0];
@@ -713,6 +746,11 @@ MHCUDAResult calc_batch_locked(MinhashCudaGenerator *gen, uint32_t count,
     uint32_t *dc = (uint32_t *)buf.cols.ptr, *dr = (uint32_t *)buf.rows.ptr,
              *dh = (uint32_t *)buf.hashes.ptr;
     std::vector<wmh::MatrixDesc> dev(nspans);
+    // the piece's uploads: (device destination, host source, bytes); pinned
+    // sources are DMA'd directly, pageable ones first copied (in parallel)
+    // into the set's pinned staging buffer
+    std::vector<wmh::CopyTask> uploads;
+    uploads.reserve(3 * nspans);
     uint64_t nnz_off = 0, row_off = 0, out_off = 0;
     for (size_t i = 0; i < nspans; i++) {
       const Span &sp = pieces[p][i];
@@ -721,22 +759,40 @@ MHCUDAResult calc_batch_locked(MinhashCudaGenerator *gen, uint32_t count,
       const uint64_t n = x.rows[sp.r1] - first;
       const uint32_t len = sp.r1 - sp.r0;
       if (n) {
-        HIPCH(wmh::copy_checked(gen, dw + nnz_off, x.weights + first, n * 4,
-                                hipMemcpyHostToDevice, gen->copy_stream),
-              mhcudaMemoryCopyError);
-        HIPCH(wmh::copy_checked(gen, dc + nnz_off, x.cols + first, n * 4,
-                                hipMemcpyHostToDevice, gen->copy_stream),
-              mhcudaMemoryCopyError);
+        uploads.push_back(wmh::CopyTask{dw + nnz_off, x.weights + first, n * 4});
+        uploads.push_back(wmh::CopyTask{dc + nnz_off, x.cols + first, n * 4});
       }
-      HIPCH(wmh::copy_checked(gen, dr + row_off, x.rows + sp.r0, ((uint64_t)len + 1) * 4,
-                              hipMemcpyHostToDevice, gen->copy_stream),
-            mhcudaMemoryCopyError);
+      uploads.push_back(wmh::CopyTask{dr + row_off, x.rows + sp.r0, ((uint64_t)len + 1) * 4});
       dev[i] = wmh::MatrixDesc{dw + nnz_off, dc + nnz_off, dr + row_off, dh + out_off,
                                len, first, 0u, 0u, 0u, 0u};
       nnz_off += n;
       row_off += (uint64_t)len + 1;
       out_off += (uint64_t)len * S2;
     }
+    std::vector<char> staged(uploads.size());
+    size_t stage_bytes = 0;
+    for (size_t i = 0; i < uploads.size(); i++) {
+      staged[i] = !wmh::is_pinned(uploads[i].src);
+      if (staged[i]) stage_bytes += (uploads[i].bytes + 63) & ~(size_t)63;
+    }
+    if (stage_bytes) {
+      // the set's previous uploads out of the staging buffer completed before
+      // its `done` event
+      if (!buf.staging.reserve(stage_bytes)) return mhcudaMemoryAllocationFailure;
+      std::vector<wmh::CopyTask> to_stage;
+      size_t at = 0;
+      for (size_t i = 0; i < uploads.size(); i++) {
+        if (!staged[i]) continue;
+        to_stage.push_back(wmh::CopyTask{buf.staging.ptr + at, uploads[i].src, uploads[i].bytes});
+        uploads[i].src = buf.staging.ptr + at;
+        at += (uploads[i].bytes + 63) & ~(size_t)63;
+      }
+      copier.copy(to_stage.data(), to_stage.size());
+    }
+    for (const wmh::CopyTask &u : uploads)
+      HIPCH(wmh::copy_checked(gen, u.dst, u.src, u.bytes, hipMemcpyHostToDevice,
+                              gen->copy_stream),
+            mhcudaMemoryCopyError);
     HIPCH(hipEventRecord(buf.uploaded, gen->copy_stream), mhcudaRuntimeError);
     HIPCH(hipStreamWaitEvent(gen->stream, buf.uploaded, 0), mhcudaRuntimeError);
     MHCUDAResult r =
@@ -772,7 +828,7 @@ MHCUDAResult calc_batch_locked(MinhashCudaGenerator *gen, uint32_t count,
   }
   HIPCH(hipStreamSynchronize(gen->d2h_stream), mhcudaRuntimeError);
   HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
-  for (auto &b : gen->pieces) b.settle();
+  for (auto &b : gen->pieces) b.settle(copier);
   INFO("mhcuda - success\n");
   return mhcudaSuccess;
 }
@@ -804,7 +860,8 @@ MHCUDAResult mhcuda_calc_batch(const MinhashCudaGenerator *cgen, uint32_t count,
   HIPCH(hipSetDevice(gen->device), mhcudaNoSuchDevice);
   DEBUG("mhcuda_calc: %u matrices, %" PRIu64 " rows, %" PRIu64 " nonzeros\n", count,
         total_rows, total_nnz);
-  const MHCUDAResult r = calc_batch_locked(gen, count, mats, kernel, max_row);
+  const MHCUDAResult r =
+      calc_batch_locked(gen, count, mats, kernel, max_row, total_nnz, total_rows);
   if (r != mhcudaSuccess) {
     // a call that failed midway may have left uploads, hashing and downloads
     // in flight on the generator's streams: drain them, so that the next call

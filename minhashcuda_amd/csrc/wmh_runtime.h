@@ -9,9 +9,11 @@
 #include <atomic>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
+#include "host_copy.h"
 #include "minhashcuda.h"
 #include "wmh_kernels.h"
 
@@ -63,20 +65,17 @@ inline bool is_pinned(const void *p) {
 }
 
 // A copy still owed from a pinned landing zone to a caller's output array.
-struct PendingCopy {
-  void *dst;
-  const void *src;
-  size_t bytes;
-};
+using PendingCopy = CopyTask;
 
 // One device buffer set of the pipelined host-memory path, with the events
-// that order its upload, hashing and download.  Hashes bound for pageable
-// outputs land in `landing` (a download into pageable memory would block the
-// host until the hashing finished) and are copied out when the set is next
-// reused or the call ends.
+// that order its upload, hashing and download.  Pageable inputs are copied
+// into `staging` (by the generator's parallel copier) and uploaded from there;
+// hashes bound for pageable outputs land in `landing` (a download into
+// pageable memory would block the host until the hashing finished) and are
+// copied out when the set is next reused or the call ends.
 struct PieceBuffers {
   DevBuf weights, cols, rows, hashes;
-  PinnedBuf landing;
+  PinnedBuf staging, landing;
   std::vector<PendingCopy> owed;
   hipEvent_t uploaded = nullptr, hashed = nullptr, done = nullptr;
   hipError_t create() {
@@ -93,8 +92,8 @@ struct PieceBuffers {
   }
   // Completes the copies owed by the previous use of this set (its `done`
   // event must have completed).
-  void settle() {
-    for (const PendingCopy &c : owed) std::memcpy(c.dst, c.src, c.bytes);
+  void settle(HostCopier &copier) {
+    copier.copy(owed.data(), owed.size());
     owed.clear();
   }
 };
@@ -193,6 +192,7 @@ struct MinhashCudaGenerator_ {
   // host-memory calls: two device buffer sets used alternately, their own
   // upload and download streams, and the size limits of one piece
   wmh::PieceBuffers pieces[2];
+  std::unique_ptr<wmh::HostCopier> copier;  // host-memory paths (created on first use)
   hipStream_t copy_stream = nullptr, d2h_stream = nullptr;
   uint64_t piece_nnz = 1ull << 28;
   uint64_t piece_rows = 1ull << 20;
