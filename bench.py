@@ -897,6 +897,24 @@ def main(argv=None):
     })
     if tuned is not None:
         result["autotune"] = tuned
+    if args.workload == "plain":
+        # the reference's own entry point on the same matrix: numpy arrays in
+        # host memory through libMHCUDA.minhash_cuda_calc (uploads, hashing,
+        # downloads), PCIe-inclusive
+        import numpy as np
+        engine.L.minhash_cuda_calc(engine.gen, m)
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            host_h = engine.L.minhash_cuda_calc(engine.gen, m)
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[len(times) // 2]
+        result["host_path"] = {
+            "value": m.nnz * SAMPLES / t, "ms_per_call": t * 1e3, "calls": len(times),
+            "unit": "nnz*samples/s (PCIe-inclusive: numpy inputs and output, "
+                    "libMHCUDA.minhash_cuda_calc, median call)",
+            "identical_to_device_resident": bool(np.array_equal(
+                host_h, outs[0].cpu().numpy().view(np.uint32)))}
     if args.workload == "batched" and not args.no_plain:
         # the plain kernel is timed on fewer steps (it is about 5x slower and
         # its launches vary little)
