@@ -31,13 +31,19 @@ struct CopyTask {
 class HostCopier {
  public:
   // Worker threads for the CPUs this process may run on: at most 7 besides
-  // the caller, none on a single CPU.
+  // the caller, none on a single CPU (fewer if the system refuses threads:
+  // the caller copies alone then).
   HostCopier() {
     cpu_set_t set;
     unsigned cpus = 1;
     if (sched_getaffinity(0, sizeof(set), &set) == 0) cpus = (unsigned)CPU_COUNT(&set);
     const unsigned workers = std::min(7u, cpus > 1 ? cpus - 1 : 0u);
-    for (unsigned i = 0; i < workers; i++) threads_.emplace_back([this] { work(); });
+    try {
+      threads_.reserve(workers);
+      for (unsigned i = 0; i < workers; i++) threads_.emplace_back([this] { work(); });
+    } catch (...) {
+      // keep the workers that started
+    }
   }
   HostCopier(const HostCopier &) = delete;
   HostCopier &operator=(const HostCopier &) = delete;
@@ -50,13 +56,20 @@ class HostCopier {
     for (auto &t : threads_) t.join();
   }
 
-  void copy(const CopyTask *tasks, size_t n) {
-    auto job = std::make_shared<Job>();
-    for (size_t i = 0; i < n; i++) {
-      char *d = static_cast<char *>(tasks[i].dst);
-      const char *s = static_cast<const char *>(tasks[i].src);
-      for (size_t off = 0; off < tasks[i].bytes; off += kChunk)
-        job->chunks.push_back(CopyTask{d + off, s + off, std::min(kChunk, tasks[i].bytes - off)});
+  void copy(const CopyTask *tasks, size_t n) noexcept {
+    std::shared_ptr<Job> job;
+    try {
+      job = std::make_shared<Job>();
+      for (size_t i = 0; i < n; i++) {
+        char *d = static_cast<char *>(tasks[i].dst);
+        const char *s = static_cast<const char *>(tasks[i].src);
+        for (size_t off = 0; off < tasks[i].bytes; off += kChunk)
+          job->chunks.push_back(
+              CopyTask{d + off, s + off, std::min(kChunk, tasks[i].bytes - off)});
+      }
+    } catch (...) {  // no memory for the chunk list: copy on this thread
+      for (size_t i = 0; i < n; i++) std::memcpy(tasks[i].dst, tasks[i].src, tasks[i].bytes);
+      return;
     }
     if (job->chunks.empty()) return;
     if (job->chunks.size() > 1 && !threads_.empty()) {
