@@ -564,7 +564,6 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
 #endif
 constexpr uint32_t kCodeGroups = WMH_CODE_GROUPS;
 
-
 // Buffer offset past every column block (block sizes are far below 2^31):
 // an unmarked lane's gather returns zero and reads no memory.
 constexpr uint32_t kNoGather = 0x80000000u;
