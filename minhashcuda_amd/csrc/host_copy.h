@@ -26,8 +26,8 @@ struct CopyTask {
 
 // A fixed set of worker threads; copy() splits the tasks into chunks that the
 // workers and the calling thread take from a shared counter, and returns when
-// every chunk is copied.  One copy() at a time (the caller serializes: the
-// generator's lock).
+// every chunk is copied.  Several threads may call copy() at once (the serving
+// loop's slots): the workers help the oldest job that still has chunks.
 class HostCopier {
  public:
   // Worker threads for the CPUs this process may run on: at most 7 besides
@@ -72,25 +72,26 @@ class HostCopier {
       return;
     }
     if (job->chunks.empty()) return;
-    if (job->chunks.size() > 1 && !threads_.empty()) {
+    const bool shared = job->chunks.size() > 1 && !threads_.empty();
+    if (shared) {
       {
         std::lock_guard<std::mutex> lock(mu_);
-        job_ = job;
-        generation_++;
+        jobs_.push_back(job);
       }
       cv_work_.notify_all();
     }
     run(*job);
     std::unique_lock<std::mutex> lock(mu_);
     cv_done_.wait(lock, [&] { return job->done.load() == job->chunks.size(); });
-    job_.reset();
+    if (shared) jobs_.erase(std::find(jobs_.begin(), jobs_.end(), job));
   }
 
  private:
-  static constexpr size_t kChunk = 2u << 20;
+  static constexpr size_t kChunk = 512u << 10;
   struct Job {
     std::vector<CopyTask> chunks;
     std::atomic<size_t> next{0}, done{0};
+    bool has_chunks() const { return next.load() < chunks.size(); }
   };
 
   void run(Job &job) {
@@ -104,16 +105,20 @@ class HostCopier {
     }
   }
 
+  // the oldest listed job with chunks left (mu_ held), or null
+  std::shared_ptr<Job> open_job() const {
+    for (const auto &j : jobs_)
+      if (j->has_chunks()) return j;
+    return nullptr;
+  }
+
   void work() {
-    uint64_t seen = 0;
     for (;;) {
       std::shared_ptr<Job> job;
       {
         std::unique_lock<std::mutex> lock(mu_);
-        cv_work_.wait(lock, [&] { return stop_ || (job_ && generation_ != seen); });
+        cv_work_.wait(lock, [&] { return stop_ || (job = open_job()) != nullptr; });
         if (stop_) return;
-        seen = generation_;
-        job = job_;
       }
       run(*job);
     }
@@ -122,8 +127,7 @@ class HostCopier {
   std::vector<std::thread> threads_;
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
-  std::shared_ptr<Job> job_;
-  uint64_t generation_ = 0;
+  std::vector<std::shared_ptr<Job>> jobs_;  // jobs the workers may help (mu_)
   bool stop_ = false;
 };
 

@@ -110,6 +110,7 @@ constexpr int kStageBlocks = 4;
 
 struct StageRing {
   wmh::PinnedBuf mem;
+  wmh::HostCopier *copier = nullptr;  // the server's (copies out of the blocks)
   hipEvent_t ev[kStageBlocks] = {};
   int next = 0;      // block taken after the current one
   int cur = -1;      // block being filled (-1: none)
@@ -135,9 +136,14 @@ struct StageRing {
   char *block(int b) { return mem.ptr + (size_t)b * kStageBlock; }
   // copies out the downloads landed in block b (its event must be complete)
   void settle(int b) {
-    for (const Landing &l : owed[b]) std::memcpy(l.dst, block(b) + l.at, l.bytes);
+    std::vector<wmh::CopyTask> out;
+    out.reserve(owed[b].size());
+    for (const Landing &l : owed[b]) out.push_back(wmh::CopyTask{l.dst, block(b) + l.at, l.bytes});
+    copier->copy(out.data(), out.size());
     owed[b].clear();
   }
+  // true when the next room() opens another block (closing the current one)
+  bool switching() const { return cur < 0 || fill == kStageBlock; }
   // closes the current block: its event marks the end of its copies
   bool close(hipStream_t s) {
     if (cur < 0) return true;
@@ -194,6 +200,15 @@ struct Slot {
   StageRing ring;
   uint64_t pool_off = 0, pool_bytes = 0;  // this group's range of the pool
   std::vector<Request *> reqs;
+  // pageable inputs staged into the current ring block, not yet copied or
+  // uploaded: (device destination, staging block, host source, bytes)
+  struct Staged {
+    char *dev;
+    char *stage;
+    const char *src;
+    size_t bytes;
+  };
+  std::vector<Staged> staged;
   Phases phases;
   bool busy = false;      // requests assigned and not yet retired
   std::thread worker;
@@ -206,6 +221,7 @@ struct MHCUDAServer_ {
   MinhashCudaGenerator *gen = nullptr;
   MHCUDAServeConfig cfg{};
   wmh::DevBuf pool;        // the memory budget, shared by every slot
+  wmh::HostCopier copier;  // staging copies of every slot, in parallel
   PoolRanges ranges;       // its free ranges (guarded by mu)
   std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
@@ -231,21 +247,38 @@ struct MHCUDAServer_ {
   }
 
   // Uploads `n` bytes from host `src` to device `dst` on the slot's stream:
-  // pinned sources directly, pageable ones through the staging ring.
+  // pinned sources directly, pageable ones through the staging ring (their
+  // copies into a block are made together, in parallel, before the block is
+  // closed or the launch is enqueued: flush_staged).
   bool upload(Slot *slot, char *dst, const char *src, size_t n, bool pinned) {
     if (pinned) return wmh::copy_checked(gen, dst, src, n, hipMemcpyHostToDevice,
                                          slot->stream) == hipSuccess;
     for (size_t done = 0; done < n;) {
+      if (slot->ring.switching() && !flush_staged(slot)) return false;
       size_t part = 0;
       char *stage = slot->ring.room(n - done, &part, slot->stream);
       if (!stage) return false;
-      std::memcpy(stage, src + done, part);
-      if (wmh::copy_checked(gen, dst + done, stage, part, hipMemcpyHostToDevice,
-                            slot->stream) != hipSuccess)
-        return false;
+      slot->staged.push_back(Slot::Staged{dst + done, stage, src + done, part});
       done += part;
     }
     return true;
+  }
+
+  // Copies the staged pageable inputs into their ring block (in parallel) and
+  // enqueues their uploads.
+  bool flush_staged(Slot *slot) {
+    if (slot->staged.empty()) return true;
+    std::vector<wmh::CopyTask> in;
+    in.reserve(slot->staged.size());
+    for (const Slot::Staged &st : slot->staged)
+      in.push_back(wmh::CopyTask{st.stage, st.src, st.bytes});
+    copier.copy(in.data(), in.size());
+    bool ok = true;
+    for (const Slot::Staged &st : slot->staged)
+      ok = ok && wmh::copy_checked(gen, st.dev, st.stage, st.bytes, hipMemcpyHostToDevice,
+                                   slot->stream) == hipSuccess;
+    slot->staged.clear();
+    return ok;
   }
 
   // Downloads `n` bytes of hashes from device `src` to host `dst`: pinned
@@ -301,6 +334,7 @@ struct MHCUDAServer_ {
                                       (const uint32_t *)dr, (uint32_t *)dh, r->length, first,
                                       0u, 0u, 0u, 0u});
     }
+    if (!flush_staged(slot)) return mhcudaMemoryCopyError;
     slot->phases.uploaded = Clock::now();
     // (untimed: the generator's timing state belongs to the calls that hold
     // its lock; serving keeps its own latency statistics)
@@ -336,6 +370,7 @@ struct MHCUDAServer_ {
         (void)hipStreamSynchronize(slot->stream);
         (void)slot->ring.close(slot->stream);
         slot->ring.reset();
+        slot->staged.clear();
       }
       const Clock::time_point now = Clock::now();
       lock.lock();
@@ -460,6 +495,7 @@ MHCUDAServer *mhcuda_serve_start(MinhashCudaGenerator *gen, const MHCUDAServeCon
         hipEventCreateWithFlags(&s->finished, hipEventDisableTiming) != hipSuccess)
       return fail_after(mhcudaRuntimeError);
     if (!s->ring.create()) return fail_after(mhcudaMemoryAllocationFailure);
+    s->ring.copier = &srv->copier;
   }
   {
     // from here on, calls that change the tables or the tuning are refused

@@ -1,14 +1,17 @@
 // CPU check of the engine's parallel host copier (minhashcuda_amd/csrc/
-// host_copy.h): copies of many shapes -- empty, one byte, around the 2 MiB
-// chunk size, tens of MiB, many small tasks at odd offsets -- must reproduce
-// their sources byte for byte, call after call (the workers' job hand-off),
-// from one copier and from a copier created and destroyed per call.  Built
-// with g++ (also under ThreadSanitizer) by tests/test_host_copy.py; prints
-// "ok" and exits 0 on success.
+// host_copy.h): copies of many shapes -- empty, one byte, around the chunk
+// size, tens of MiB, many small tasks at odd offsets -- must reproduce their
+// sources byte for byte, call after call (the workers' job hand-off), from
+// several threads calling one copier at once (the serving loop's slots), and
+// from a copier created and destroyed per call.  Built with g++ (also under
+// ThreadSanitizer) by tests/test_host_copy.py; prints "ok" and exits 0 on
+// success.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "host_copy.h"
@@ -54,6 +57,23 @@ int main(int argc, char **argv) {
         return 1;
       }
     }
+  }
+  // four callers at once on one copier
+  std::atomic<int> bad{0};
+  std::vector<std::thread> callers;
+  for (int t = 0; t < 4; t++) {
+    callers.emplace_back([&, t] {
+      std::mt19937_64 trng(100 + t);
+      for (int r = 0; r < rounds; r++) {
+        if (!run_case(copier, trng, 1 + trng() % 12, 3u << 20)) bad++;
+        if (!run_case(copier, trng, 50, 3000)) bad++;
+      }
+    });
+  }
+  for (auto &c : callers) c.join();
+  if (bad.load() != 0) {
+    std::printf("mismatch: %d concurrent calls\n", bad.load());
+    return 1;
   }
   for (int r = 0; r < 8; r++) {  // copiers created and destroyed around one call
     wmh::HostCopier once;
