@@ -1,9 +1,10 @@
 """GPU tests of the host-side paths around the kernels: calls on different
-streams sharing one generator's launch staging, armed allocation and copy
-failures of the pipelined host path and of serving slots (the error reaches
-the caller, and the generator or server stays usable), admission of requests
-against the whole serving budget, and the refusal of parameter or tuning
-changes while a server runs."""
+streams sharing one generator's launch staging, host-memory calls mixing
+pinned and pageable arrays (pageable ones are staged through pinned buffers
+by the copy pool), armed allocation and copy failures of the pipelined host
+path and of serving slots (the error reaches the caller, and the generator
+or server stays usable), admission of requests against the whole serving
+budget, and the refusal of parameter or tuning changes while a server runs."""
 import ctypes
 import os
 
@@ -66,6 +67,60 @@ def test_batches_on_two_streams_share_the_staging_safely(gen_params, capi):
         torch.cuda.synchronize()
         for m, buf in list(zip(a, bufs_a)) + list(zip(b, bufs_b)):
             assert np.array_equal(buf[3].cpu().numpy().view(np.uint32), reference(params, m)), rnd
+
+
+def pinned_like(a):
+    """A copy of numpy array `a` in pinned host memory (and the tensor that owns it)."""
+    t = torch.empty(a.shape, dtype=torch.int32 if a.dtype != np.float32 else torch.float32,
+                    pin_memory=True)
+    v = t.numpy().view(a.dtype)
+    v[...] = a
+    return t, v
+
+
+@pytest.mark.parametrize("pieces", ["auto", "many"])
+def test_host_calls_mix_pinned_and_pageable_arrays(libmhcuda, capi, gpu_available, pieces):
+    """mhcuda_calc_batch over matrices whose weights, columns, indptr and
+    output are each pinned or pageable in every combination (pageable inputs
+    go through the staging buffers, pinned ones straight to DMA; pageable
+    outputs through the landing buffers), in one piece or in pieces of at
+    most 3,000 nonzeros: every hash equals the oracle's."""
+    if pieces == "many":
+        os.environ["MHCUDA_PIECE_NNZ"] = "3000"
+    try:
+        gen = libmhcuda.minhash_cuda_init(DIM, SAMPLES, seed=43, devices=1)
+    finally:
+        os.environ.pop("MHCUDA_PIECE_NNZ", None)
+    g = ctypes.c_void_p(gen)
+    try:
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+        rng = np.random.default_rng(8)
+        keep, mats, outs, refs = [], [], [], []
+        for combo in range(16):
+            m = workloads.batched_csr(rng.integers(1, 600, 40 + combo).astype(np.int64),
+                                      dim=DIM, seed=50 + combo)
+            arrays = [m.data.astype(np.float32), m.indices.astype(np.uint32),
+                      m.indptr.astype(np.uint32),
+                      np.zeros((m.shape[0], SAMPLES, 2), np.uint32)]
+            for bit in range(4):
+                if combo >> bit & 1:
+                    t, arrays[bit] = pinned_like(arrays[bit])
+                    keep.append(t)
+            keep.append(arrays)
+            mats.append(capi_mod.Matrix(arrays[0].ctypes.data, arrays[1].ctypes.data,
+                                        arrays[2].ctypes.data, m.shape[0],
+                                        arrays[3].ctypes.data))
+            outs.append(arrays[3])
+            refs.append(reference(params, m))
+        descs = (capi_mod.Matrix * len(mats))(*mats)
+        for kernel in (capi_mod.KERNEL_PLAIN, capi_mod.KERNEL_PERSISTENT):
+            for o in outs:
+                o[...] = 0
+            assert capi.mhcuda_calc_batch(g, len(mats), descs, kernel) == 0
+            for combo, (o, ref) in enumerate(zip(outs, refs)):
+                assert np.array_equal(o, ref), (kernel, combo)
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
 
 
 def test_pipelined_path_failures_propagate_and_recover(libmhcuda, capi, gpu_available):
