@@ -75,6 +75,8 @@ def parse(argv=None):
     p.add_argument("--wpc", type=int, default=0, help="persistent waves per CU (0: default)")
     p.add_argument("--stripe-mib", type=int, default=-1,
                    help="persistent column-stripe size in MiB (0: off, -1: library default)")
+    p.add_argument("--seeds", type=int, default=-1,
+                   help="heavy seeds of striped rows (0: off, 1: on, -1: library default)")
     p.add_argument("--no-skip", action="store_true",
                    help="hash without the skip bound (mhcuda_set_skip(0))")
     p.add_argument("--no-plain", action="store_true", help="batched: skip the plain comparison")
@@ -225,6 +227,8 @@ class Engine:
             capi_mod.check(self.capi.mhcuda_set_stripes(self.g, args.stripe_mib), "stripes")
         if getattr(args, "no_skip", False):
             capi_mod.check(self.capi.mhcuda_set_skip(self.g, 0), "skip")
+        if getattr(args, "seeds", -1) >= 0:
+            capi_mod.check(self.capi.mhcuda_set_seeds(self.g, args.seeds), "seeds")
         capi_mod.check(self.capi.mhcuda_set_timing(self.g, 1), "timing")
         capi_mod.check(self.capi.mhcuda_set_graph(self.g, 0 if args.no_graph else 1), "graph")
         self.graph = not args.no_graph
@@ -236,9 +240,12 @@ class Engine:
         self.capi.mhcuda_get_stripes(self.g, ctypes.byref(mib), ctypes.byref(k), ctypes.byref(n))
         en, act = ctypes.c_int(), ctypes.c_int()
         self.capi.mhcuda_get_skip(self.g, ctypes.byref(en), ctypes.byref(act))
+        sd = ctypes.c_int()
+        self.capi.mhcuda_get_seeds(self.g, ctypes.byref(sd))
         return {"samples_per_lane": a.value, "segment": b.value, "waves_per_cu": c.value,
                 "stripe_mib": mib.value, "stripes": k.value, "striped_row_min_nnz": n.value,
-                "skip_bound": bool(act.value), "hip_graph": self.graph}
+                "skip_bound": bool(act.value), "heavy_seeds": bool(sd.value),
+                "hip_graph": self.graph}
 
     def kernel_ms(self):
         ms = ctypes.c_float()
@@ -256,10 +263,10 @@ class Engine:
 # ---------------------------------------------------------------------------
 # rooflines from the committed rocprofv3 counter records
 
-def counter_record(workload, kernel, alg_bytes, stripes=None):
+def counter_record(workload, kernel, alg_bytes, stripes=None, seeds=False):
     """The rocprofv3 record (profiles/hbm_traffic.json) of `kernel` on this
     workload's input (identified by its algorithmic bytes) at this stripe
-    count, or None."""
+    count and heavy-seed setting, or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
@@ -270,6 +277,8 @@ def counter_record(workload, kernel, alg_bytes, stripes=None):
         if rec.get("kernel") != kernel or rec.get("algorithmic_bytes") != alg_bytes:
             continue
         if stripes is not None and rec.get("stripes") != stripes:
+            continue
+        if bool(rec.get("seeds", False)) != bool(seeds):
             continue
         return rec
     return None
@@ -285,18 +294,20 @@ def measured_traffic(workload, kernel, alg_bytes, stripes=None):
     return None if rec is None else rec.get("traffic_bytes")
 
 
-def roofline(workload, kname, alg_bytes, avg_ms, stripes):
+def roofline(workload, kname, alg_bytes, avg_ms, stripes, seeds=False):
     """The dominant kernel against the resource that binds it.  Two ceilings
-    are computed from the counter record of the same kernel, input and stripe
-    count: VALU issue (SQ_INSTS_VALU x 64 lane operations per launch over the
-    launch time, against VALU_PEAK_LANE_OPS) and HBM (FETCH_SIZE + WRITE_SIZE
-    bytes per launch over the launch time, against 8 TB/s).  `bound` is the
+    are computed from the counter record of the same kernel, input, stripe
+    count and heavy-seed setting: VALU issue (SQ_INSTS_VALU x 64 lane
+    operations per launch over the launch time, against VALU_PEAK_LANE_OPS)
+    and HBM (FETCH_SIZE + WRITE_SIZE bytes per launch over the launch time,
+    against 8 TB/s).  `bound` is the
     one closer to its peak; the algorithmic bytes over the launch time are
     reported beside it as hbm_algorithmic_x (above 1 where the caches serve
     re-read parameters)."""
     t = avg_ms * 1e-3
-    rec = counter_record(workload, kname, alg_bytes,
-                         stripes if kname == "wmh_persistent_kernel" else None)
+    persistent = kname == "wmh_persistent_kernel"
+    rec = counter_record(workload, kname, alg_bytes, stripes if persistent else None,
+                         seeds and persistent)
     traffic = None if rec is None else rec.get("traffic_bytes")
     insts = None if rec is None else rec.get("valu_insts")
     hbm_frac = traffic / t / (HBM_PEAK_GBS * 1e9) if traffic else None
@@ -325,7 +336,7 @@ def roofline(workload, kname, alg_bytes, avg_ms, stripes):
 
 def explicit_tuning(args):
     """True when the command line fixes a tuning knob (then nothing is autotuned)."""
-    return bool(args.spl or args.seg or args.wpc) or args.stripe_mib >= 0
+    return bool(args.spl or args.seg or args.wpc) or args.stripe_mib >= 0 or args.seeds >= 0
 
 
 def autotune(engine, descs, stream):
@@ -754,7 +765,8 @@ def large_run(args, torch, steps, warmup, with_cpu):
                                     "(18.4 GB); value = the autotuned hashing kernel on one "
                                     "resident 65,536-row chunk (1.1e9 nnz)",
                         "chunk_rows": chunk_rows, "chunk_nnz": nnz}, **tuning),
-        "roofline": roofline("large", kname, alg, avg_ms, tuning["stripes"]),
+        "roofline": roofline("large", kname, alg, avg_ms, tuning["stripes"],
+                             tuning["heavy_seeds"]),
         "end_to_end": {"rows": total_rows, "nnz": e2e_nnz,
                        "seconds": min(ends), "value": e2e_units / min(ends),
                        "unit": "nnz*samples/s (PCIe-inclusive, pinned host input)",
@@ -883,7 +895,7 @@ def main(argv=None):
     avg_ms = sum(kms) / len(kms)
     alg = workloads.algorithmic_bytes(nnz, rows, SAMPLES)
     tuning = engine.tuning()
-    roof = roofline(args.workload, kname, alg, avg_ms, tuning["stripes"])
+    roof = roofline(args.workload, kname, alg, avg_ms, tuning["stripes"], tuning["heavy_seeds"])
     roof["timed_span"] = ("hipGraph replay: descriptor upload, plan, init, %s, fix-up (HIP "
                           "events around the graph launch)" % kname
                           if engine.graph and kernel == cm.KERNEL_PERSISTENT else

@@ -186,6 +186,18 @@ MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib
 MHCUDAResult mhcuda_set_skip(MinhashCudaGenerator *gen, int enable);
 MHCUDAResult mhcuda_get_skip(const MinhashCudaGenerator *gen, int *enabled, int *active);
 
+/* Heavy seeds of the column-stripe pieces (off by default; the autotuner may
+ * turn them on): before the persistent kernel, one wavefront per (striped
+ * row, 128-sample tile) hashes 64 heavy nonzeros of the row (the heaviest of
+ * each 64th position) and stores their minima, so that every stripe piece,
+ * the row's first included, takes the skip bound from its first nonzero on.
+ * Used when the skip bound is active, at 2 samples per lane, with a sample
+ * count divisible by 8 and more than one stripe.  Pays on rows whose weights
+ * spread widely (the heavy nonzeros then hold most minima); the hashes do not
+ * depend on it.  No counterpart in the reference. */
+MHCUDAResult mhcuda_set_seeds(MinhashCudaGenerator *gen, int enable);
+MHCUDAResult mhcuda_get_seeds(const MinhashCudaGenerator *gen, int *enabled);
+
 /* Result of mhcuda_autotune. */
 typedef struct {
   uint64_t key;              /* samples | p50 bin << 16 | p90 bin << 24 | size class << 32 */
@@ -199,6 +211,7 @@ typedef struct {
   uint64_t sample_nnz;       /* nonzeros of the timed prefix */
   int cached;                /* 1: taken from the cache, nothing was timed */
   int stripe_mib;            /* column-stripe size (0: no stripes) */
+  int seeds;                 /* heavy seeds of the striped rows (mhcuda_set_seeds) */
 } MHCUDATuneResult;
 
 /* Autotunes the kernel choice and tuning for a batch of device-resident
@@ -206,7 +219,7 @@ typedef struct {
  * device, and unless a batch of the same shape (histogram bins of the median
  * and 90th-percentile nonzero, size class, samples) was tuned before, times
  * candidates (kernel, samples per lane, item size, waves per CU, column-stripe
- * size) on a prefix of the batch holding up to 2^32 nonzeros.  The choice
+ * size, heavy seeds) on a prefix of the batch holding up to 2^32 nonzeros.  The choice
  * becomes the generator's tuning (kernel mhcudaKernelAuto then uses it).  The
  * timed runs write the prefix's hashes into its outputs. */
 MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t count,

@@ -23,7 +23,7 @@ EXPORTS = (
     "mhcuda_assign_random_vars", "mhcuda_calc", "mhcuda_fini",
     "mhcuda_calc_device", "mhcuda_calc_device_noalloc", "mhcuda_calc_batch_device", "mhcuda_calc_batch",
     "mhcuda_set_tuning", "mhcuda_get_tuning", "mhcuda_set_stripes", "mhcuda_get_stripes",
-    "mhcuda_set_skip", "mhcuda_get_skip",
+    "mhcuda_set_skip", "mhcuda_get_skip", "mhcuda_set_seeds", "mhcuda_get_seeds",
     "mhcuda_autotune", "mhcuda_jaccard_pairs",
     "mhcuda_lsh_band_keys", "mhcuda_lsh_candidates", "mhcuda_save_random_vars", "mhcuda_load_random_vars",
     "mhcuda_params_file_info", "mhcuda_set_graph",
@@ -54,7 +54,7 @@ class TuneResult(ctypes.Structure):
                 ("samples_per_lane", ctypes.c_int), ("segment", ctypes.c_int),
                 ("waves_per_cu", ctypes.c_int), ("sample_ms", ctypes.c_float),
                 ("sample_nnz", ctypes.c_uint64), ("cached", ctypes.c_int),
-                ("stripe_mib", ctypes.c_int)]
+                ("stripe_mib", ctypes.c_int), ("seeds", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -113,6 +113,10 @@ def load(path):
     L.mhcuda_get_stripes.restype = i32
     L.mhcuda_get_stripes.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(u32),
                                      ctypes.POINTER(u32)]
+    L.mhcuda_set_seeds.restype = i32
+    L.mhcuda_set_seeds.argtypes = [vp, i32]
+    L.mhcuda_get_seeds.restype = i32
+    L.mhcuda_get_seeds.argtypes = [vp, ctypes.POINTER(i32)]
     L.mhcuda_set_skip.restype = i32
     L.mhcuda_set_skip.argtypes = [vp, i32]
     L.mhcuda_get_skip.restype = i32

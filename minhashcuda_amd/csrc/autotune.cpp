@@ -92,6 +92,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
     res.segment = gen->tuning.seg;
     res.waves_per_cu = gen->tuning.waves_per_cu;
     res.stripe_mib = gen->tuning.stripe_mib;
+    res.seeds = gen->tuning.seeds;
     res.cached = 1;
     if (result) *result = res;
     return mhcudaSuccess;
@@ -197,16 +198,25 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
         c.tuning.waves_per_cu = wpc;
         consider(c);
       }
-      // column stripes (only where the table spans several stripes)
+      // column stripes (only where the table spans several stripes), with
+      // and without heavy seeds where the stripe pieces can use them (the
+      // best stripe size differs: seeded pieces gather less)
       const Candidate after_wpc = best;
+      const bool seedable = wmh::launch_bounds_of(gen).codes != nullptr &&
+                            gen->samples % 8 == 0 && after_wpc.tuning.spl == 2;
       uint32_t k = 1, min_nnz = 0;
-      for (int mib : {0, 4, 8, 16, 32, 64, 256}) {
-        if (mib == after_wpc.tuning.stripe_mib) continue;
-        wmh::stripe_shape(gen->dim, gen->samples, mib, &k, &min_nnz);
-        if (mib != 0 && k < 2) continue;
-        Candidate c = after_wpc;
-        c.tuning.stripe_mib = mib;
-        consider(c);
+      for (int seeds : {after_wpc.tuning.seeds, 1 - after_wpc.tuning.seeds}) {
+        if (seeds != after_wpc.tuning.seeds && !seedable) continue;
+        for (int mib : {0, 4, 8, 16, 32, 64, 256}) {
+          if (mib == after_wpc.tuning.stripe_mib && seeds == after_wpc.tuning.seeds) continue;
+          wmh::stripe_shape(gen->dim, gen->samples, mib, &k, &min_nnz);
+          if (mib != 0 && k < 2) continue;
+          if (mib == 0 && seeds != after_wpc.tuning.seeds) continue;  // seeds need stripes
+          Candidate c = after_wpc;
+          c.tuning.stripe_mib = mib;
+          c.tuning.seeds = seeds;
+          consider(c);
+        }
       }
     }
     (void)hipEventDestroy(e0);
@@ -216,9 +226,9 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
     if (r != mhcudaSuccess) return r;
     if (verbosity > 0)
       printf("autotune key %llx: kernel %d spl %d seg %d waves/CU %d stripes %d MiB "
-             "(%.3f ms on %llu nnz)\n",
+             "seeds %d (%.3f ms on %llu nnz)\n",
              (unsigned long long)key, best.kernel, best.tuning.spl, best.tuning.seg,
-             best.tuning.waves_per_cu, best.tuning.stripe_mib, best_ms,
+             best.tuning.waves_per_cu, best.tuning.stripe_mib, best.tuning.seeds, best_ms,
              (unsigned long long)have);
     wmh::TunedChoice choice;
     choice.kernel = best.kernel;
@@ -236,6 +246,7 @@ extern "C" MHCUDAResult mhcuda_autotune(MinhashCudaGenerator *gen, uint32_t coun
   res.segment = it->second.tuning.seg;
   res.waves_per_cu = it->second.tuning.waves_per_cu;
   res.stripe_mib = it->second.tuning.stripe_mib;
+  res.seeds = it->second.tuning.seeds;
   res.sample_ms = it->second.ms;
   res.sample_nnz = it->second.sample_nnz;
   if (result) *result = res;

@@ -329,7 +329,7 @@ MHCUDAResult hash_graph(MinhashCudaGenerator *gen, std::vector<MatrixDesc> &mats
                     gc.tuning.spl == gen->tuning.spl && gc.tuning.seg == gen->tuning.seg &&
                     gc.tuning.waves_per_cu == gen->tuning.waves_per_cu &&
                     gc.tuning.stripe_mib == gen->tuning.stripe_mib &&
-                    gc.fast_div == gen->fast_div &&
+                    gc.tuning.seeds == gen->tuning.seeds && gc.fast_div == gen->fast_div &&
                     same_table(gc.bounds, launch_bounds_of(gen)) && gc.key.size() == mats.size() &&
                     std::memcmp(gc.key.data(), mats.data(), sizeof(MatrixDesc) * mats.size()) == 0;
   if (!same) {
@@ -980,6 +980,20 @@ MHCUDAResult mhcuda_set_stripes(MinhashCudaGenerator *gen, int stripe_mib) {
   std::lock_guard<std::mutex> lock(gen->mu);
   if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tuning
   gen->tuning.stripe_mib = stripe_mib;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_set_seeds(MinhashCudaGenerator *gen, int enable) {
+  if (!gen) return mhcudaInvalidArguments;
+  std::lock_guard<std::mutex> lock(gen->mu);
+  if (gen->servers.load() > 0) return mhcudaInvalidArguments;  // a server reads the tuning
+  gen->tuning.seeds = enable != 0;
+  return mhcudaSuccess;
+}
+
+MHCUDAResult mhcuda_get_seeds(const MinhashCudaGenerator *gen, int *enabled) {
+  if (!gen || !enabled) return mhcudaInvalidArguments;
+  *enabled = gen->tuning.seeds;
   return mhcudaSuccess;
 }
 

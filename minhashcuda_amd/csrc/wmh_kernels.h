@@ -52,6 +52,7 @@ struct Tuning {
   int seg;          // nonzeros per persistent work item (multiple of 64)
   int waves_per_cu; // persistent grid size = CUs * waves_per_cu
   int stripe_mib;   // parameter bytes per column stripe in MiB (0: no stripes)
+  int seeds = 0;    // 1: heavy seeds give striped rows their first minima
 };
 
 // Column stripes of the persistent launch.  When the parameter table is

@@ -564,9 +564,9 @@ __device__ __forceinline__ void flush_row(uint32_t *__restrict__ output, uint32_
 #endif
 constexpr uint32_t kCodeGroups = WMH_CODE_GROUPS;
 
-// Buffer offset past every column block (block sizes are far below 2^31):
-// an unmarked lane's gather returns zero and reads no memory.
-constexpr uint32_t kNoGather = 0x80000000u;
+// Buffer offsets with the top bit set lie past every column block (block
+// sizes are far below 2^31): an unmarked lane's gather returns zero and reads
+// no memory.
 
 // Packed 0x4800 - T' of the lane's V running minima, two samples per word
 // (wmh_bound.h).  At one sample per lane the high half is never tested.
@@ -781,10 +781,14 @@ __device__ __forceinline__ void hash_item(const float *__restrict__ params, uint
             lwu[u] = readlane_f32(lw, i + u);
             const __amdgpu_buffer_rsrc_t rs = block_rsrc(
                 pbase + (uint64_t)(j[u] < dim ? j[u] : 0u) * block_bytes, block_bytes);
-            const bool g = !Bounded || ((gm >> u) & 1u) != 0u;
-            load_b<V, kStream>(rs, g ? lo.r : kNoGather, rr[u]);
-            load_b<V, kStream>(rs, g ? lo.beta : kNoGather, beta[u]);
-            load_b<V, kStream>(rs, g ? lo.lc : kNoGather, lc[u]);
+            // an unmarked lane's offsets get the top bit (arithmetic, not a
+            // select: the compiler turns a select of load offsets into a
+            // branch around each load, with a wait for every load in flight
+            // at the join)
+            const uint32_t nog = Bounded ? (~(gm >> u) & 1u) << 31 : 0u;
+            load_b<V, kStream>(rs, lo.r | nog, rr[u]);
+            load_b<V, kStream>(rs, lo.beta | nog, beta[u]);
+            load_b<V, kStream>(rs, lo.lc | nog, lc[u]);
           }
           if (inside) {
 #pragma unroll
@@ -1375,6 +1379,143 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
   }
 }
 
+// Weights in flight per lane of the seed scan (16-byte runs) and seeds whose
+// gathers are issued together (8 runs, 16 seeds, or 6 workgroups per CU
+// instead of 4 measured the same: the kernel is bound by HBM).
+constexpr int kSeedRuns = 4;
+constexpr int kSeedGroup = 8;
+constexpr int kSeedBlocksPerCu = 4;
+
+// Heavy seeds of the striped rows: one wavefront per (striped row, 128-sample
+// tile), after the init pass and before the persistent kernel, when the
+// stripe pieces use the skip bound (2 samples per lane, samples % 8 == 0).
+// Lane l takes the heaviest nonzero of its share of the row (largest weight,
+// the earliest of equal ones, NaN never; a column >= dim leaves the lane
+// without a seed), so the 64 seeds are heavy nonzeros spread over the whole
+// row.  They are hashed exactly for the tile's samples (2 per lane,
+// kernel.cu:103-106), keeping per sample the smallest (ln_a, position) -- the
+// seeds are not in CSR order, so equal values keep the earlier position
+// explicitly -- and stored as keyed minima in the row's slots, which the init
+// pass set to kNoKey and which only this wavefront writes before the
+// persistent kernel.  Every stripe piece of the row then finds minima for its
+// tile and goes through the skip bound from its first nonzero on: a
+// nonzero's chance to hold a sample's minimum grows with its weight, so heavy
+// seeds start the bound near the row's final minima.  The seeds are real
+// candidates of the row and the merge orders by (ln_a, position), so no hash
+// changes (a seed's own position is evaluated again by its piece, with the
+// same key).  Costs one read of the striped rows' weights and 64 nonzeros per
+// row and tile (bound by HBM: the seeds' parameter gathers move about twice
+// the weights' bytes).
+__global__ __launch_bounds__(kBlock) void batch_seed_kernel(
+    const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
+    const MatrixDesc *__restrict__ mats, const BatchCtl *__restrict__ ctl,
+    StripePlan stripes, uint32_t fast_div) {
+  WMH_NO_CONTRACT
+  if (uniform(ctl->invalid)) return;
+  const uint32_t lane = lane_id();
+  const uint32_t ntiles = (samples + 127u) / 128u;
+  const uint64_t units = (uint64_t)uniform(ctl->n_striped) * ntiles;
+  const uint32_t block_bytes = 12u * sp;
+  const char *pbase = reinterpret_cast<const char *>(params);
+  for (uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); w < units;
+       w += (uint64_t)gridDim.x * kWavesPerBlock) {
+    const uint32_t idx = uniform((uint32_t)(w / ntiles));
+    const uint32_t tile = uniform((uint32_t)(w - (uint64_t)idx * ntiles));
+    const StripedRow sr = stripes.list[idx];
+    const uint32_t m = uniform(sr.matrix), row = uniform(sr.row);
+    const MatrixDesc M = mats[m];
+    const uint32_t s = uniform(M.rows[row]), e = uniform(M.rows[row + 1]);
+    const float *__restrict__ weights = M.weights - M.base;
+    const uint32_t *__restrict__ cols = M.cols - M.base;
+    if (s >= e) continue;
+    // The lane's heaviest nonzero of its runs of 4 positions: the runs are
+    // 16-byte aligned from the one holding position s on, lane l takes runs
+    // l, l + 64, ..., kSeedRuns of them in flight.  Positions outside [s, e)
+    // that share a run with s or with a position of the row are read (the
+    // same 16 bytes, so mapped memory) but never taken.  Runs and positions
+    // go up, so the strict > keeps the earliest of equal weights.
+    float bw = -__builtin_inff();
+    uint32_t bp = WMH_SENTINEL;
+    const int64_t a0 = (int64_t)s - (int64_t)((reinterpret_cast<uintptr_t>(weights + s) >> 2) & 3u);
+    for (int64_t q0 = a0; q0 < (int64_t)e; q0 += 4 * kSeedRuns * kWave) {
+      float4 x[kSeedRuns];
+#pragma unroll
+      for (int u = 0; u < kSeedRuns; u++) {
+        const int64_t q = q0 + 4 * (int64_t)(lane + u * kWave);
+        x[u] = q < (int64_t)e ? *reinterpret_cast<const float4 *>(weights + q)
+                              : make_float4(-__builtin_inff(), -__builtin_inff(),
+                                            -__builtin_inff(), -__builtin_inff());
+      }
+#pragma unroll
+      for (int u = 0; u < kSeedRuns; u++) {
+        const int64_t q = q0 + 4 * (int64_t)(lane + u * kWave);
+        const float xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const bool take = q + k >= (int64_t)s && q + k < (int64_t)e && xs[k] > bw;
+          bw = take ? xs[k] : bw;
+          bp = take ? (uint32_t)(q + k) : bp;
+        }
+      }
+    }
+    uint32_t c = dim;
+    if (bp != WMH_SENTINEL) c = cols[bp];
+    const bool ok = c < dim;
+    const float lw = ok ? wmh_logf(bw) : __builtin_nanf("");
+    const bool fast = fast_div != 0u && __ballot(lw == __builtin_inff()) == 0ull;
+    const uint32_t k0 = tile * 128u + 2u * lane;
+    const uint32_t koff = k0 < samples ? k0 : 0u;
+    const LaneOffsets lo{koff * 4u, (sp + koff) * 4u, (2u * sp + koff) * 4u};
+    float best[2] = {FLT_MAX, FLT_MAX};
+    uint32_t pos[2] = {WMH_SENTINEL, WMH_SENTINEL};
+    const unsigned long long have = __ballot(ok);
+    with_division(fast, [&](auto mode) {
+      constexpr bool kFast = decltype(mode)::value;
+      constexpr int U = kSeedGroup;
+      for (uint32_t i = 0; i < (uint32_t)kWave; i += U) {
+        if (((have >> i) & ((1ull << U) - 1ull)) == 0ull) continue;
+        float rr[U][2], beta[U][2], lc[U][2];
+        float lwu[U];
+        uint32_t pu[U];
+        bool oku[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const uint32_t j = readlane_u32(c, i + u);
+          oku[u] = ((have >> (i + u)) & 1ull) != 0ull;
+          lwu[u] = readlane_f32(lw, i + u);
+          pu[u] = readlane_u32(bp, i + u);
+          const __amdgpu_buffer_rsrc_t rs =
+              block_rsrc(pbase + (uint64_t)(oku[u] ? j : 0u) * block_bytes, block_bytes);
+          load_b<2>(rs, lo.r, rr[u]);
+          load_b<2>(rs, lo.beta, beta[u]);
+          load_b<2>(rs, lo.lc, lc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+          for (int v = 0; v < 2; v++) {
+            const float t = __builtin_floorf(div_lw<kFast>(lwu[u], rr[u][v]) + beta[u][v]);
+            const float ln_y = (t - beta[u][v]) * rr[u][v];
+            const float ln_a = (lc[u][v] - ln_y) - rr[u][v];
+            const bool upd = oku[u] && (ln_a < best[v] || (ln_a == best[v] &&
+                                                           pos[v] != WMH_SENTINEL &&
+                                                           pu[u] < pos[v]));
+            best[v] = upd ? ln_a : best[v];
+            pos[v] = upd ? pu[u] : pos[v];
+          }
+        }
+      }
+    });
+    unsigned long long *slot =
+        reinterpret_cast<unsigned long long *>(M.output) + ((uint64_t)row * samples + k0);
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      if (k0 + v < samples && pos[v] != WMH_SENTINEL)
+        slot[v] = ((unsigned long long)order_key(best[v]) << 32) | pos[v];
+    }
+  }
+}
+
 // Item plan (one workgroup): the item size, the item prefix over the
 // matrices, and the reset of the work queue.  With `target_items` > 0 the
 // item size is the batch's nonzeros / target_items rounded down to a multiple
@@ -1826,6 +1967,12 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   const unsigned row_grid = grid_for(total_chunks);
   hipLaunchKernelGGL(batch_rows_kernel<kInit>, dim3(row_grid), dim3(kBlock), 0, stream,
                      params, samples, sp, mats, count, total_chunks, ctl, stripes);
+  // heavy seeds where the stripe pieces use the skip bound
+  if (tuning.seeds && codes != nullptr && samples % 8u == 0u && spl == 2 && stripes.stripes > 1)
+    hipLaunchKernelGGL(batch_seed_kernel,
+                       dim3((unsigned)std::max(1, num_cus * kSeedBlocksPerCu)),
+                       dim3(kBlock), 0, stream, params, dim, samples, sp, mats, ctl, stripes,
+                       (uint32_t)fast_div);
   const unsigned grid = (unsigned)std::max(1, num_cus * tuning.waves_per_cu / kWavesPerBlock);
   if (ev_begin) (void)hipEventRecord(ev_begin, stream);
   switch (spl) {

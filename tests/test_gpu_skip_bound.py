@@ -208,3 +208,50 @@ def test_row_walk_bound_with_stripes_and_short_rows(libmhcuda, capi, gpu_availab
             assert np.array_equal(got, ref), spl
     finally:
         libmhcuda.minhash_cuda_fini(gen)
+
+
+@pytest.mark.parametrize("samples", [128, 256, 64, 200])
+def test_heavy_seeds_equal_the_oracle(libmhcuda, capi, gpu_available, samples):
+    """Heavy seeds of the striped rows (mhcuda_set_seeds): the same hashes as
+    without them and as the oracle, for one and two 128-sample tiles and part
+    tiles, with special weights, columns >= dim, a striped row of zeros only
+    (no seed has a finite hash), a striped row whose every nonzero has a
+    column >= dim, and duplicate (column, weight) pairs of the heaviest weight
+    in one lane and in neighbouring lanes (equal hashes at two positions: the
+    earlier one wins)."""
+    print("samples", samples, flush=True)
+    dim = 1 << 15
+    rng = np.random.default_rng(samples + 3)
+    nnz = np.concatenate([rng.integers(1100, 9000, 10), rng.integers(1, 300, 20),
+                          [20000, 1500, 1300]])
+    m = workloads.batched_csr(nnz.astype(np.int64), dim=dim + 9, sigma=2.0, seed=samples + 4)
+    m = special_weights(m, rng, 500)
+    lens = np.diff(m.indptr)
+    zeros_row, far_row = len(nnz) - 2, len(nnz) - 1
+    m.data[m.indptr[zeros_row]:m.indptr[zeros_row + 1]] = 0.0
+    m.indices[m.indptr[far_row]:m.indptr[far_row + 1]] = dim + 3
+    dup_row = int(np.flatnonzero(lens >= 1100)[1])  # (row 0 holds the infinite weight)
+    s = int(m.indptr[dup_row])
+    for a, b in ((5, 69), (300, 301)):
+        m.indices[s + b] = m.indices[s + a]
+        m.data[s + a] = m.data[s + b] = np.float32(3.3e38)
+    gen = libmhcuda.minhash_cuda_init(dim, samples, seed=samples + 5, devices=1)
+    g = ctypes.c_void_p(gen)
+    try:
+        params = libmhcuda.minhash_cuda_retrieve_vars(gen)
+        ref = oracle.calc(*params, m.data, m.indices, m.indptr)
+        assert capi.mhcuda_set_stripes(g, 1) == 0
+        assert skip_state(capi, g) == (1, 1)
+        sd = ctypes.c_int()
+        assert capi.mhcuda_get_seeds(g, ctypes.byref(sd)) == 0 and sd.value == 0
+        assert capi.mhcuda_set_seeds(g, 1) == 0
+        assert capi.mhcuda_get_seeds(g, ctypes.byref(sd)) == 0 and sd.value == 1
+        for seg in (512, 4096):
+            assert capi.mhcuda_set_tuning(g, 2, seg, 0) == 0
+            got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+            assert np.array_equal(got, ref), seg
+        assert capi.mhcuda_set_seeds(g, 0) == 0
+        off = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+        assert np.array_equal(off, ref)
+    finally:
+        libmhcuda.minhash_cuda_fini(gen)
