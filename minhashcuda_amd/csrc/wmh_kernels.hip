@@ -999,8 +999,8 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
 //   order) for every sample and cannot change the merged result.
 //   Phase 2: each lane evaluates its marked nonzeros exactly (8 samples, the
 //   formula of kernel.cu:103-106) in CSR order, one per round; lanes with
-//   nothing left to evaluate issue no loads.  The thresholds then follow the
-//   quarter's new minima.
+//   nothing left to evaluate issue no loads.  Every kThresholdEvery-th
+//   chunk the thresholds then follow the quarter's new minima.
 // The four quarters merge by (ln_a, position) and lanes 0-15 merge the result
 // into the row's keyed minima like every piece of a split row.  Requires
 // samples % 8 == 0 (every lane's 8 samples lie in its column's runs).
@@ -1031,26 +1031,37 @@ __device__ __forceinline__ void code_thresholds(const float (&thr)[8], float sca
 }
 
 // Folds samples H*4 .. H*4 + 3 of the lane's 8 (one 16-byte load of each
-// parameter array) for the nonzero at position p into the running minima.
+// parameter array) for the nonzero at position p into the running minima;
+// true when one of them changed.
 template <bool Fast, int H>
-__device__ __forceinline__ void fold_half(const float4 &r4, const float4 &b4, const float4 &c4,
-                                          float lw, uint32_t p, bool on, float (&best)[8],
+__device__ __forceinline__ bool fold_half(const float4 &r4, const float4 &b4, const float4 &c4,
+                                          float lw, uint32_t p, float (&best)[8],
                                           uint32_t (&pos)[8]) {
   WMH_NO_CONTRACT
   const float r[4] = {r4.x, r4.y, r4.z, r4.w};
   const float beta[4] = {b4.x, b4.y, b4.z, b4.w};
   const float lc[4] = {c4.x, c4.y, c4.z, c4.w};
+  bool any = false;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int v = 4 * H + i;
     const float t = __builtin_floorf(div_lw<Fast>(lw, r[i]) + beta[i]);  // kernel.cu:103
     const float ln_y = (t - beta[i]) * r[i];              // kernel.cu:104
     const float ln_a = (lc[i] - ln_y) - r[i];             // kernel.cu:105
-    const bool upd = on && ln_a < best[v];                // kernel.cu:106
+    const bool upd = ln_a < best[v];                      // kernel.cu:106
     best[v] = upd ? ln_a : best[v];
     pos[v] = upd ? p : pos[v];
+    any |= upd;
   }
+  return any;
 }
+
+// Chunks between refreshes of the code thresholds from the running minima.
+// Thresholds of older minima are higher: the test then skips a little less,
+// never a pair that could win (a skipped pair stays above an existing
+// candidate).  Minima change in few chunks once the row holds good ones, and
+// a refresh costs about as much as a quarter of the chunk's tests.
+constexpr uint32_t kThresholdEvery = 4;
 
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
                                                    const uint8_t *__restrict__ codes,
@@ -1092,6 +1103,8 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
   }
   uint32_t tpair[4];
   code_thresholds(thr, code_scale, code_t0, tpair);
+  bool stale = false;  // the lane's minima moved below its thresholds
+  uint32_t chunk_no = 0;
   // the next chunk's columns and weights are loaded while this one is tested
   uint32_t next_c = dim;
   float next_w = 0.0f;
@@ -1155,35 +1168,35 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     }
     const uint32_t live = n > quarter ? (n - quarter + 3u) >> 2 : 0u;  // positions of this quarter
     marked &= lane_ok ? (1u << live) - 1u : 0u;
-    const bool any_marked = __ballot(marked != 0u) != 0ull;
-  // phase 2: exact evaluation of the marked nonzeros, lowest position first
-    auto phase2 = [&](auto mode) {
-      for (;;) {
-        const bool has = marked != 0u;
-        if (__ballot(has) == 0ull) break;
-        const uint32_t m = has ? (uint32_t)__builtin_ctz(marked) : 0u;
-        marked &= marked - 1u;
-        const uint32_t src = 4u * m + quarter;
-        const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
-        constexpr bool kFast = decltype(mode)::value;
-        // the lane's two halves of 4 samples one after the other (all 8 at
-        // once would hold 24 loaded values and cost a wave per SIMD)
-        const uint32_t ou = bperm_u32(o, src) + 4u * koff;
-        if (has) {
-          float4 r4 = load4(rs, ou, 0u), b4 = load4(rs, ou, s_b), c4 = load4(rs, ou, s_c);
-          fold_half<kFast, 0>(r4, b4, c4, lws, cb + src, true, best, pos);
-          r4 = load4(rs, ou + 16u, 0u);
-          b4 = load4(rs, ou + 16u, s_b);
-          c4 = load4(rs, ou + 16u, s_c);
-          fold_half<kFast, 1>(r4, b4, c4, lws, cb + src, true, best, pos);
+    // phase 2: exact evaluation of the marked nonzeros, lowest position
+    // first (the division mode is picked per round: a round waits for its
+    // loads before its folds anyway)
+    for (;;) {
+      const bool has = marked != 0u;
+      if (__ballot(has) == 0ull) break;
+      const uint32_t m = has ? (uint32_t)__builtin_ctz(marked) : 0u;
+      marked &= marked - 1u;
+      const uint32_t src = 4u * m + quarter;
+      const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
+      const uint32_t ou = bperm_u32(o, src) + 4u * koff;
+      if (has) {
+        const float4 r0 = load4(rs, ou, 0u), b0 = load4(rs, ou, s_b), c0 = load4(rs, ou, s_c);
+        const float4 r1 = load4(rs, ou + 16u, 0u), b1 = load4(rs, ou + 16u, s_b),
+                     c1 = load4(rs, ou + 16u, s_c);
+        if (fast) {
+          stale |= fold_half<true, 0>(r0, b0, c0, lws, cb + src, best, pos);
+          stale |= fold_half<true, 1>(r1, b1, c1, lws, cb + src, best, pos);
+        } else {
+          stale |= fold_half<false, 0>(r0, b0, c0, lws, cb + src, best, pos);
+          stale |= fold_half<false, 1>(r1, b1, c1, lws, cb + src, best, pos);
         }
       }
-    };
-    if (any_marked) {
-      with_division(fast, [&](auto mode) { phase2(mode); });
+    }
+    if (++chunk_no % kThresholdEvery == 0u && __ballot(stale) != 0ull) {
 #pragma unroll
       for (int v = 0; v < 8; v++) thr[v] = best[v] < thr[v] ? best[v] : thr[v];
       code_thresholds(thr, code_scale, code_t0, tpair);
+      stale = false;
     }
   }
   // merge the quarters by (ln_a, position): lanes l, l ^ 16, l ^ 32, l ^ 48
