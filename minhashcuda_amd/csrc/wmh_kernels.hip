@@ -1061,7 +1061,7 @@ __device__ __forceinline__ bool fold_half(const float4 &r4, const float4 &b4, co
 // never a pair that could win (a skipped pair stays above an existing
 // candidate).  Minima change in few chunks once the row holds good ones, and
 // a refresh costs about as much as a quarter of the chunk's tests.
-constexpr uint32_t kThresholdEvery = 4;
+constexpr uint32_t kThresholdEvery = 8;
 
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
                                                    const uint8_t *__restrict__ codes,

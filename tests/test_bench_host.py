@@ -38,11 +38,15 @@ def test_measured_traffic_lookup(tmp_path, monkeypatch):
                "traffic_bytes": 900.0},
               {"kernel": "k_persistent", "algorithmic_bytes": 1000, "stripes": 64,
                "traffic_bytes": 250.0},
+              {"kernel": "k_persistent", "algorithmic_bytes": 1000, "stripes": 64,
+               "seeds": True, "traffic_bytes": 200.0},
               {"kernel": "k_plain", "algorithmic_bytes": 1000, "traffic_bytes": 1010.0}]}}
     (tmp_path / "profiles").mkdir()
     (tmp_path / "profiles" / "hbm_traffic.json").write_text(json.dumps(recs))
     monkeypatch.setattr(bench, "ROOT", str(tmp_path))
     assert bench.measured_traffic("w", "k_persistent", 1000, 64) == 250.0
+    assert bench.measured_traffic("w", "k_persistent", 1000, 64, seeds=True) == 200.0
+    assert bench.measured_traffic("w", "k_persistent", 1000, 1, seeds=True) is None
     assert bench.measured_traffic("w", "k_persistent", 1000, 1) == 900.0
     assert bench.measured_traffic("w", "k_persistent", 1000, 32) is None
     assert bench.measured_traffic("w", "k_persistent", 999, 64) is None
@@ -106,10 +110,11 @@ def test_committed_lines_keep_the_contract(name):
     assert ROOFLINE_KEYS <= set(roof)
     assert roof["bound"] in ("valu", "hbm") and 0 < roof["frac"] <= 1
     # the line's roofline is what bench.roofline computes from the committed
-    # counter record of the same kernel, input and stripe count
+    # counter record of the same kernel, input, stripe count and seed setting
     stripes = line["config"].get("stripes")
     again = bench.roofline(name, roof["kernel"], roof["algorithmic_bytes_per_launch"],
-                           roof["avg_kernel_ms"], stripes)
+                           roof["avg_kernel_ms"], stripes,
+                           line["config"].get("heavy_seeds", False))
     for key in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
         assert again[key] == pytest.approx(roof[key]) if isinstance(roof[key], float) \
             else again[key] == roof[key], key
@@ -137,11 +142,12 @@ def test_default_line_carries_the_large_share_and_serving():
 @pytest.mark.parametrize("name", ["batched", "plain", "large"])
 def test_committed_traffic_matches_records(name):
     """The roofline's traffic in each committed line is the rocprofv3 record
-    of the same kernel, input and stripe count."""
+    of the same kernel, input, stripe count and heavy-seed setting."""
     line = load_line(name)
     roof = line["roofline"]
-    stripes = (line["config"]["stripes"] if roof["kernel"] == "wmh_persistent_kernel"
-               else None)
+    persistent = roof["kernel"] == "wmh_persistent_kernel"
+    stripes = line["config"]["stripes"] if persistent else None
+    seeds = persistent and line["config"].get("heavy_seeds", False)
     traffic = bench.measured_traffic(name, roof["kernel"],
-                                     roof["algorithmic_bytes_per_launch"], stripes)
+                                     roof["algorithmic_bytes_per_launch"], stripes, seeds)
     assert traffic is not None and traffic == roof["traffic"]
