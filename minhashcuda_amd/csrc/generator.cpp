@@ -300,7 +300,7 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen) {
       float lo = 0.0f, delta = 0.0f;
       wmh_code_range(mn, mx, &lo, &delta);
       HIPCH(launch_build_codes(gen->params, gen->dim, gen->samples, lo, delta, gen->bounds,
-                               gen->stream),
+                               gen->coarse, gen->stream),
             mhcudaRuntimeError);
       HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
       gen->code_scale = 1.0f / delta;
@@ -468,10 +468,13 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
   // engine hashes without it when it cannot be allocated or is switched off
   const char *skip_env = std::getenv("MHCUDA_SKIP_BOUND");
   if (!(skip_env && std::strcmp(skip_env, "0") == 0)) {
-    if (hipMalloc((void **)&gen->bounds, (size_t)dim * gen->sp) != hipSuccess) {
+    if (hipMalloc((void **)&gen->bounds, (size_t)wmh::code_table_bytes(dim, samples)) !=
+        hipSuccess) {
       (void)hipGetLastError();
       gen->bounds = nullptr;
       INFO("no memory for the bound table: hashing without the skip bound\n");
+    } else if (gen->sp % 8u == 0u) {
+      gen->coarse = gen->bounds + (size_t)dim * gen->sp;
     }
   }
   if (deferred) {

@@ -136,12 +136,21 @@ hipError_t launch_params_range(const float *params, uint32_t dim, uint32_t sampl
 
 // The byte-coded bound table of the skip bound (wmh_bound.h): `codes`
 // [dim][sp] bytes, and the scale 1 / delta and offset t0 = 2049 - lo / delta
-// of its thresholds.  codes == null: the skip bound is not used.
+// of its thresholds.  codes == null: the skip bound is not used.  `coarse`
+// [dim][sp / 8] holds the smallest code of every 8-sample group (null unless
+// sp % 8 == 0); the bounded stripe pieces test it before the 8 codes.
 struct CodeTable {
   const uint8_t *codes;
   float scale;
   float t0;
+  const uint8_t *coarse = nullptr;
 };
+
+// Bytes of the bound table with its coarse codes for dim x samples.
+inline uint64_t code_table_bytes(uint32_t dim, uint32_t samples) {
+  const uint64_t sp = padded_samples(samples);
+  return (uint64_t)dim * sp + (sp % 8u == 0u ? (uint64_t)dim * (sp / 8u) : 0u);
+}
 
 // Smallest and largest bound entry (wmh_bound_entry) over the real samples
 // of the table, as order-preserving keys (wmh_order_bits) into minmax[0]
@@ -151,9 +160,10 @@ hipError_t launch_bound_range(const float *params, uint32_t dim, uint32_t sample
                               uint32_t *minmax, hipStream_t stream);
 
 // Fills codes [dim][sp] with wmh_bound_code of every entry for the given lo
-// and delta (padded samples: 255).
+// and delta (padded samples: 255), then, when coarse is not null (sp % 8 ==
+// 0), coarse [dim][sp / 8] with the smallest of every 8 consecutive codes.
 hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t samples, float lo,
-                              float delta, uint8_t *codes, hipStream_t stream);
+                              float delta, uint8_t *codes, uint8_t *coarse, hipStream_t stream);
 
 // Scatter one [samples][ncols] FP32 table (rows of pitch ncols) into
 // params[(j0 + jj) * 3 * sp + which * sp + k].

@@ -240,6 +240,28 @@ __global__ __launch_bounds__(kBlock) void build_codes_kernel(const float *__rest
   }
 }
 
+// coarse[i] = the smallest of codes[8i .. 8i + 7]: per column, the smallest
+// code of each 8-sample group (a column's sp codes start at a multiple of 8
+// when sp % 8 == 0).  A group whose smallest code is skipped against the
+// group's largest threshold T' has every code skipped against its own
+// threshold: the skip test (wmh_code_skips) only turns on as c grows or T'
+// falls.
+__global__ __launch_bounds__(kBlock) void build_coarse_kernel(const uint8_t *__restrict__ codes,
+                                                              uint64_t groups,
+                                                              uint8_t *__restrict__ coarse) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < groups;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const uint2 w = *reinterpret_cast<const uint2 *>(codes + 8u * i);
+    uint32_t m = 255u;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      m = min(m, (w.x >> (8 * b)) & 0xffu);
+      m = min(m, (w.y >> (8 * b)) & 0xffu);
+    }
+    coarse[i] = (uint8_t)m;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The hashing core.
 
@@ -985,18 +1007,21 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
 
 // Stripe pieces of a 128-sample tile with the skip bound (wmh_bound.h), for
 // pieces whose row already holds a keyed minimum for every sample of the
-// tile (the earlier stripes' pieces wrote them).  Quarter-wave layout: lane
-// l hashes samples k0 = 128 tile + 8 (l % 16) .. k0 + 7 at the positions
-// cb + 4m + l / 16 (m = 0..15) of each 64-position chunk, so one 8-byte load
-// per lane covers four nonzeros x 128 samples of the byte-coded bound table.
+// tile (heavy seeds or the earlier stripes' pieces wrote them).  Quarter-wave
+// layout: lane l hashes samples k0 = 128 tile + 8 (l % 16) .. k0 + 7 at the
+// positions cb + 4m + l / 16 (m = 0..15) of each 64-position chunk.
 //   Phase 1: for every (nonzero, sample) the code c is tested against the
 //   sample's threshold T' -- from the smaller of the row's keyed minimum read
 //   at the start and the quarter's own minimum -- and the nonzero's term
 //   (wmh_code_skips, two samples per 32-bit add): a lane marks nonzero m
-//   when any of its 8 samples is not skipped.  A skipped (nonzero, sample)
-//   has a computed ln_a above the threshold, so an unmarked nonzero is above
-//   the row's minimum (or above the quarter's, which comes earlier in CSR
-//   order) for every sample and cannot change the merged result.
+//   when any of its 8 samples is not skipped.  First the coarse test, one
+//   byte per (nonzero, lane): the smallest of the 8 codes against the largest
+//   of the 8 thresholds; a group it skips has every sample skipped.  Then the
+//   8 codes of the groups it kept (a few percent once the row holds good
+//   minima).  A skipped (nonzero, sample) has a computed ln_a above the
+//   threshold, so an unmarked nonzero is above the row's minimum (or above
+//   the quarter's, which comes earlier in CSR order) for every sample and
+//   cannot change the merged result.
 //   Phase 2: each lane evaluates its marked nonzeros exactly (8 samples, the
 //   formula of kernel.cu:103-106) in CSR order, one per round; lanes with
 //   nothing left to evaluate issue no loads.  Every kThresholdEvery-th
@@ -1011,8 +1036,14 @@ __device__ __forceinline__ float key_value(unsigned long long key) {
   return __uint_as_float((h & 0x80000000u) ? (h & 0x7fffffffu) : ~h);
 }
 
-// Phase-1 steps whose code loads are issued together (all 16 of a chunk
-// need 147 VGPRs and lose a wave per SIMD).
+// Fine tests per lane and round whose code loads are issued together (1, 2,
+// 4, 6 and 8 measured 189.1 / 180.5 / 181.9 / 186.8 / 193.2 ms on the uneven
+// batch: one per round waits for a round trip per test, larger batches test
+// for idle lanes).
+constexpr int kFineBatch = 2;
+
+// Phase-1 steps whose coarse code loads are issued together (the loop over
+// the two groups of a chunk stays rolled: unrolled, it needs 151 VGPRs).
 constexpr int kBoundGroup = 8;
 
 // Two byte codes of a 32-bit word (bytes 2i, 2i + 1) as 16-bit halves.
@@ -1021,13 +1052,19 @@ __device__ __forceinline__ uint32_t code_pair(uint32_t w) {
   return __builtin_amdgcn_perm(0u, w, I == 0 ? 0x0c010c00u : 0x0c030c02u);
 }
 
-// Packed 0x4800 - T' of the lane's 8 samples, two per word (wmh_bound.h).
+// Packed 0x4800 - T' of the lane's 8 samples, two per word (wmh_bound.h),
+// and 0x4800 - max T' in both halves of tmax2 (the coarse test).
 __device__ __forceinline__ void code_thresholds(const float (&thr)[8], float scale, float t0,
-                                                uint32_t (&tpair)[4]) {
+                                                uint32_t (&tpair)[4], uint32_t &tmax2) {
+  uint32_t tmax = 0u;
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-    tpair[i] = (0x4800u - wmh_code_threshold(thr[2 * i], scale, t0)) |
-               ((0x4800u - wmh_code_threshold(thr[2 * i + 1], scale, t0)) << 16);
+  for (int i = 0; i < 4; i++) {
+    const uint32_t lo = wmh_code_threshold(thr[2 * i], scale, t0);
+    const uint32_t hi = wmh_code_threshold(thr[2 * i + 1], scale, t0);
+    tpair[i] = (0x4800u - lo) | ((0x4800u - hi) << 16);
+    tmax = max(tmax, max(lo, hi));
+  }
+  tmax2 = (0x4800u - tmax) * 0x10001u;
 }
 
 // Folds samples H*4 .. H*4 + 3 of the lane's 8 (one 16-byte load of each
@@ -1065,6 +1102,7 @@ constexpr uint32_t kThresholdEvery = 8;
 
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
                                                    const uint8_t *__restrict__ codes,
+                                                   const uint8_t *__restrict__ coarse,
                                                    float code_scale, float code_t0,
                                                    uint32_t dim, uint32_t sp, uint32_t samples,
                                                    uint32_t tile, const MatrixDesc &M, uint32_t a,
@@ -1091,6 +1129,15 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       __builtin_amdgcn_make_buffer_rsrc(pbase, (short)0, (int)0xffffffff, kRawBufferWord3);
   const __amdgpu_buffer_rsrc_t rb =
       __builtin_amdgcn_make_buffer_rsrc(bbase, (short)0, (int)0xffffffff, kRawBufferWord3);
+  // coarse codes: sp / 8 bytes per column, byte g of a column for the lane's
+  // 8-sample group
+  const uint32_t spc = sp >> 3;
+  const uint32_t gbyte = koff >> 3;
+  const uint64_t cbb = reinterpret_cast<uint64_t>(coarse) + (uint64_t)col0 * spc;
+  char *cbase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(cbb >> 32)) << 32) |
+                                         uniform((uint32_t)cbb));
+  const __amdgpu_buffer_rsrc_t rc =
+      __builtin_amdgcn_make_buffer_rsrc(cbase, (short)0, (int)0xffffffff, kRawBufferWord3);
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
   float best[8], thr[8];
@@ -1101,8 +1148,8 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     pos[v] = WMH_SENTINEL;
     thr[v] = slot_best[v];
   }
-  uint32_t tpair[4];
-  code_thresholds(thr, code_scale, code_t0, tpair);
+  uint32_t tpair[4], tmax2;
+  code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
   bool stale = false;  // the lane's minima moved below its thresholds
   uint32_t chunk_no = 0;
   // the next chunk's columns and weights are loaded while this one is tested
@@ -1140,34 +1187,65 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     if (__ballot(c - col0 > span) != 0ull) return false;
     const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
     const uint32_t rel = c - col0;
-    const uint32_t o = rel * block_bytes, ob = rel * sp;
+    const uint32_t o = rel * block_bytes, ob = rel * sp, oc = rel * spc;
     const uint32_t lw_term = wmh_code_lw(lwb, code_scale);
     const uint32_t lw_pair = lw_term | (lw_term << 16);
-    // phase 1: the lane's marked nonzeros, bit m for position cb + 4m + quarter
-    // (positions at or past n, and lanes past the samples, are masked off)
-    uint32_t marked = 0u;
+    // Phase 1a, the coarse test: bit m of cskip when the group's smallest
+    // code, tested against the lane's largest threshold, is skipped -- then
+    // all 8 of its samples are.  Steps m and m + 4 of a group of 8 share one
+    // 32-bit add (16-bit halves, as in the fine test).
     const uint32_t steps = uniform((n + 3u) >> 2);
+    uint32_t cskip = 0u;
     for (uint32_t m0 = 0; m0 < steps; m0 += kBoundGroup) {
-      uint2 x[kBoundGroup];
+      uint32_t x[kBoundGroup];
 #pragma unroll
       for (int u = 0; u < kBoundGroup; u++) {
         const uint32_t src = 4u * (m0 + u) + quarter;  // lanes past n read valid data
-        x[u] = __builtin_bit_cast(
-            uint2, __builtin_amdgcn_raw_buffer_load_b64(rb, bperm_u32(ob, src) + koff, 0, 0));
+        x[u] = __builtin_amdgcn_raw_buffer_load_b8(rc, bperm_u32(oc, src) + gbyte, 0, 0);
       }
 #pragma unroll
-      for (int u = 0; u < kBoundGroup; u++) {
-        const uint32_t lp = bperm_u32(lw_pair, 4u * (m0 + u) + quarter);
-        // bit 15 of each half: that sample skipped (wmh_code_skips)
-        const uint32_t h = (code_pair<0>(x[u].x) + tpair[0] + lp) &
-                           (code_pair<1>(x[u].x) + tpair[1] + lp) &
-                           (code_pair<0>(x[u].y) + tpair[2] + lp) &
-                           (code_pair<1>(x[u].y) + tpair[3] + lp);
-        if ((h & 0x80008000u) != 0x80008000u) marked |= 1u << (m0 + u);
+      for (int u = 0; u < kBoundGroup / 2; u++) {
+        const uint32_t lp = bperm_u32(lw_term, 4u * (m0 + u) + quarter) |
+                            (bperm_u32(lw_term, 4u * (m0 + u + 4u) + quarter) << 16);
+        // bit 15 / 31: step m0 + u / m0 + u + 4 skipped (wmh_code_skips)
+        const uint32_t h = __builtin_amdgcn_perm(x[u + 4], x[u], 0x0c040c00u) + tmax2 + lp;
+        cskip |= ((h >> (15u - (m0 + u))) & (1u << (m0 + u))) |
+                 ((h >> (27u - (m0 + u))) & (1u << (m0 + u + 4u)));
       }
     }
     const uint32_t live = n > quarter ? (n - quarter + 3u) >> 2 : 0u;  // positions of this quarter
-    marked &= lane_ok ? (1u << live) - 1u : 0u;
+    uint32_t cmarked = ~cskip & (lane_ok ? (1u << live) - 1u : 0u);
+    // Phase 1b, the fine test of the groups the coarse test kept, up to
+    // kFineBatch per lane and round with their loads in flight together (a
+    // lane with fewer reads the stripe's first column, which stays in L2):
+    // bit m of `marked` when one of the lane's 8 samples of position
+    // cb + 4m + quarter is not skipped.
+    uint32_t marked = 0u;
+    while (__ballot(cmarked != 0u) != 0ull) {
+      uint32_t mk[kFineBatch], lpk[kFineBatch];
+      uint2 xk[kFineBatch];
+#pragma unroll
+      for (int k = 0; k < kFineBatch; k++) {
+        const uint32_t has = cmarked != 0u ? 1u : 0u;
+        mk[k] = has ? (uint32_t)__builtin_ctz(cmarked) : 32u;
+        cmarked &= cmarked - 1u;
+        const uint32_t src = 4u * (mk[k] & 15u) + quarter;
+        const uint32_t ob_m = bperm_u32(ob, src) & (0u - has);  // arithmetic, not a select
+        lpk[k] = bperm_u32(lw_pair, src);
+        xk[k] = __builtin_bit_cast(
+            uint2, __builtin_amdgcn_raw_buffer_load_b64(rb, ob_m + koff, 0, 0));
+      }
+#pragma unroll
+      for (int k = 0; k < kFineBatch; k++) {
+        // bit 15 of each half: that sample skipped (wmh_code_skips)
+        const uint32_t h = (code_pair<0>(xk[k].x) + tpair[0] + lpk[k]) &
+                           (code_pair<1>(xk[k].x) + tpair[1] + lpk[k]) &
+                           (code_pair<0>(xk[k].y) + tpair[2] + lpk[k]) &
+                           (code_pair<1>(xk[k].y) + tpair[3] + lpk[k]);
+        // (bit 32 of a 64-bit one: nothing for a lane without a step)
+        if ((h & 0x80008000u) != 0x80008000u) marked |= (uint32_t)(1ull << mk[k]);
+      }
+    }
     // phase 2: exact evaluation of the marked nonzeros, lowest position
     // first (the division mode is picked per round: a round waits for its
     // loads before its folds anyway)
@@ -1195,7 +1273,7 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     if (++chunk_no % kThresholdEvery == 0u && __ballot(stale) != 0ull) {
 #pragma unroll
       for (int v = 0; v < 8; v++) thr[v] = best[v] < thr[v] ? best[v] : thr[v];
-      code_thresholds(thr, code_scale, code_t0, tpair);
+      code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
       stale = false;
     }
   }
@@ -1246,7 +1324,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
     StripePlan stripes, uint32_t fast_div, const uint8_t *__restrict__ codes, float code_scale,
-    float code_t0) {
+    float code_t0, const uint8_t *__restrict__ coarse) {
   const uint32_t lane = lane_id();
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
@@ -1350,7 +1428,7 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
           if (piece) {
             const uint64_t col0 = (uint64_t)k * stripes.width;
             const uint32_t c0 = col0 < dim ? (uint32_t)col0 : dim;
-            if (codes != nullptr && samples % 8u == 0u) {
+            if (codes != nullptr && coarse != nullptr && samples % 8u == 0u) {
               // the row's keyed minima of the lane's 8 samples (quarter
               // layout); once the earlier pieces left one for every sample
               // of the tile, the bound can skip most exact evaluations
@@ -1367,8 +1445,8 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
                 sbest[v] = key_value(key);
               }
               if (__ballot(none) == 0ull &&
-                  hash_piece_bounded(params, codes, code_scale, code_t0, dim, sp, samples,
-                                     tile, M, a, b, row, c0, fast_div != 0u, sbest))
+                  hash_piece_bounded(params, codes, coarse, code_scale, code_t0, dim, sp,
+                                     samples, tile, M, a, b, row, c0, fast_div != 0u, sbest))
                 continue;
             }
             hash_piece_pairs(params, dim, sp, samples, tile, M, a, b, row, c0, fast_div != 0u);
@@ -1923,13 +2001,20 @@ hipError_t launch_bound_range(const float *params, uint32_t dim, uint32_t sample
 }
 
 hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t samples, float lo,
-                              float delta, uint8_t *codes, hipStream_t stream) {
+                              float delta, uint8_t *codes, uint8_t *coarse, hipStream_t stream) {
   const uint32_t sp = padded_samples(samples);
   const uint64_t total = (uint64_t)dim * sp;
   uint64_t blocks = (total + kBlock - 1) / kBlock;
   if (blocks > 65536u * 4u) blocks = 65536u * 4u;
   hipLaunchKernelGGL(build_codes_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
                      dim3(kBlock), 0, stream, params, dim, samples, sp, lo, delta, codes);
+  if (coarse != nullptr) {
+    if (sp % 8u != 0u) return hipErrorInvalidValue;
+    uint64_t cblocks = (total / 8u + kBlock - 1) / kBlock;
+    if (cblocks > 65536u * 4u) cblocks = 65536u * 4u;
+    hipLaunchKernelGGL(build_coarse_kernel, dim3((unsigned)std::max<uint64_t>(1, cblocks)),
+                       dim3(kBlock), 0, stream, codes, total / 8u, coarse);
+  }
   return hipGetLastError();
 }
 
@@ -1981,7 +2066,8 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   hipLaunchKernelGGL(batch_rows_kernel<kInit>, dim3(row_grid), dim3(kBlock), 0, stream,
                      params, samples, sp, mats, count, total_chunks, ctl, stripes);
   // heavy seeds where the stripe pieces use the skip bound
-  if (tuning.seeds && codes != nullptr && samples % 8u == 0u && spl == 2 && stripes.stripes > 1)
+  if (tuning.seeds && codes != nullptr && table.coarse != nullptr && samples % 8u == 0u &&
+      spl == 2 && stripes.stripes > 1)
     hipLaunchKernelGGL(batch_seed_kernel,
                        dim3((unsigned)std::max(1, num_cus * kSeedBlocksPerCu)),
                        dim3(kBlock), 0, stream, params, dim, samples, sp, mats, ctl, stripes,
@@ -1992,17 +2078,17 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
     case 1:
       hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, codes, table.scale, table.t0);
+                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
       break;
     case 2:
       hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, codes, table.scale, table.t0);
+                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
       break;
     default:
       hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, codes, table.scale, table.t0);
+                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
       break;
   }
   if (ev_end) (void)hipEventRecord(ev_end, stream);

@@ -172,11 +172,13 @@ struct MinhashCudaGenerator_ {
   // with the short correctly rounded sequence (refresh_division_mode)
   bool fast_div = false;
   // byte-coded bound table of the skip bound ([dim][sp] bytes, wmh_bound.h;
-  // null when MHCUDA_SKIP_BOUND=0 or it could not be allocated) with the
-  // scale and offset of its thresholds, `bounded` while it matches the
-  // parameters and they pass the bound's range check, and `skip` the user's
-  // switch (mhcuda_set_skip)
+  // null when MHCUDA_SKIP_BOUND=0 or it could not be allocated) followed in
+  // the same allocation by its coarse codes (`coarse`, [dim][sp / 8] bytes,
+  // null unless sp % 8 == 0), with the scale and offset of its thresholds,
+  // `bounded` while it matches the parameters and they pass the bound's range
+  // check, and `skip` the user's switch (mhcuda_set_skip)
   uint8_t *bounds = nullptr;
+  uint8_t *coarse = nullptr;
   float code_scale = 0.0f, code_t0 = 0.0f;
   bool bounded = false;
   bool skip = true;
@@ -253,7 +255,7 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen);
 // off or unusable for these tables.
 inline CodeTable launch_bounds_of(const MinhashCudaGenerator *gen) {
   if (!(gen->skip && gen->bounded)) return CodeTable{nullptr, 0.0f, 0.0f};
-  return CodeTable{gen->bounds, gen->code_scale, gen->code_t0};
+  return CodeTable{gen->bounds, gen->code_scale, gen->code_t0, gen->coarse};
 }
 
 inline bool same_table(const CodeTable &x, const CodeTable &y) {

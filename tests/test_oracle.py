@@ -411,6 +411,37 @@ def test_code_sum_bit15_is_the_skip_test():
         assert np.array_equal((word >> (16 * half + 15)) & 1, (single >> 15) & 1)
 
 
+def test_coarse_skip_implies_every_fine_skip():
+    """The bounded stripe pieces first test the smallest code of an 8-sample
+    group against the group's largest T' and skip the group when that sum's
+    bit 15 is set.  The skip bit only turns on as c grows or T' falls, so
+    then every sample's own (code, T') pair is skipped too: checked over
+    random groups of 8 codes and thresholds (clamped ranges, ties and the
+    extremes included) against the per-sample test."""
+    rng = np.random.default_rng(11)
+    n = 200_000
+    codes = rng.integers(0, 256, (n, 8))
+    codes[: n // 4] = rng.integers(250, 256, (n // 4, 8))           # near the top
+    tp = rng.integers(0, 4352, (n, 8))
+    tp[n // 4: n // 2] = rng.integers(2040, 2060, (n // 4, 8))       # around T' = 2048
+    cb = rng.integers(-2048, 2305, n)
+    cb[::7] = rng.choice([-2048, 2304, 0], n // 7 + 1)
+    skip = lambda c, t: ((c + (0x4800 - t) + (0x4000 - cb[:, None])) >> 15) & 1
+    coarse = skip(codes.min(axis=1, keepdims=True), tp.max(axis=1, keepdims=True))[:, 0]
+    fine = skip(codes, tp)
+    assert coarse.sum() > n // 10                   # the coarse test does skip groups
+    assert np.all(fine[coarse == 1] == 1)
+    # planted mistake: the largest code in place of the smallest is not safe
+    wrong = skip(codes.max(axis=1, keepdims=True), tp.max(axis=1, keepdims=True))[:, 0]
+    assert np.any(fine[wrong == 1] == 0)
+    # the packed form: steps m and m + 4 share one add, skip bits 15 and 31
+    lo, hi = codes[:, 0], codes[:, 1]
+    tmax = tp.max(axis=1)
+    word = (lo | hi << 16) + (0x4800 - tmax) * 0x10001 + ((0x4000 - cb) * 0x10001)
+    assert np.array_equal((word >> 15) & 1, skip(lo[:, None], tmax[:, None])[:, 0])
+    assert np.array_equal((word >> 31) & 1, skip(hi[:, None], tmax[:, None])[:, 0])
+
+
 def test_codes_are_lower_bounds_of_the_entries():
     """Codes of generated-table entries and of every fp16 pattern decode to
     values at or below the entry, within one step of it unless clamped at
