@@ -1011,24 +1011,25 @@ __device__ __forceinline__ void hash_piece_pairs(const float *__restrict__ param
 // layout: lane l hashes samples k0 = 128 tile + 8 (l % 16) .. k0 + 7 at the
 // positions cb + 4m + l / 16 (m = 0..15) of each 64-position chunk.
 //   Phase 1: for every (nonzero, sample) the code c is tested against the
-//   sample's threshold T' -- from the smaller of the row's keyed minimum read
-//   at the start and the quarter's own minimum -- and the nonzero's term
-//   (wmh_code_skips, two samples per 32-bit add): a lane marks nonzero m
-//   when any of its 8 samples is not skipped.  First the coarse test, one
-//   byte per (nonzero, lane): the smallest of the 8 codes against the largest
-//   of the 8 thresholds; a group it skips has every sample skipped.  Then the
-//   8 codes of the groups it kept (a few percent once the row holds good
-//   minima).  A skipped (nonzero, sample) has a computed ln_a above the
-//   threshold, so an unmarked nonzero is above the row's minimum (or above
-//   the quarter's, which comes earlier in CSR order) for every sample and
-//   cannot change the merged result.
-//   Phase 2: each lane evaluates its marked nonzeros exactly (8 samples, the
-//   formula of kernel.cu:103-106) in CSR order, one per round; lanes with
-//   nothing left to evaluate issue no loads.  Every kThresholdEvery-th
-//   chunk the thresholds then follow the quarter's new minima.
-// The four quarters merge by (ln_a, position) and lanes 0-15 merge the result
-// into the row's keyed minima like every piece of a split row.  Requires
-// samples % 8 == 0 (every lane's 8 samples lie in its column's runs).
+//   sample's threshold T' -- from the tile's minimum: the row's keyed minimum
+//   at the start, then the piece's own -- and the nonzero's term
+//   (wmh_code_skips, two samples per 32-bit add).  First the coarse test, one
+//   byte per (nonzero, lane): the smallest of the lane's 8 codes against the
+//   largest of its 8 thresholds; a group it skips has every sample skipped.
+//   Then the 8 codes of the groups it kept (a few percent once the row holds
+//   good minima), sample by sample.  A skipped (nonzero, sample) has a
+//   computed ln_a strictly above a minimum that a real candidate of the row
+//   reached, so it cannot change the merged result.
+//   Phase 2: every (nonzero, sample) pair the fine test keeps becomes an
+//   item of an LDS list; the wavefront evaluates the list one item per lane
+//   (the formula of kernel.cu:103-106 for one sample) and merges each result
+//   into the tile's minima in LDS by an atomic minimum of the key
+//   (order(ln_a) << 32 | position), which keeps the earliest of equal
+//   values.  Every kThresholdEvery-th chunk the lanes' thresholds then follow
+//   the tile's minima, which the four quarters share.
+// At the end lanes merge the tile's minima into the row's keyed minima like
+// every piece of a split row.  Requires samples % 8 == 0 (every lane's 8
+// samples lie in its column's runs).
 
 // The value of a keyed minimum (order_key(ln_a) << 32 | position).
 __device__ __forceinline__ float key_value(unsigned long long key) {
@@ -1037,10 +1038,10 @@ __device__ __forceinline__ float key_value(unsigned long long key) {
 }
 
 // Fine tests per lane and round whose code loads are issued together (1, 2,
-// 4, 6 and 8 measured 189.1 / 180.5 / 181.9 / 186.8 / 193.2 ms on the uneven
-// batch: one per round waits for a round trip per test, larger batches test
-// for idle lanes).
-constexpr int kFineBatch = 2;
+// 3 and 4 measured 182.9 / 175.1 / 167.0 / 176.4 ms on the uneven batch:
+// fewer wait for more round trips, more test for idle lanes and take more
+// LDS for the round's items).
+constexpr int kFineBatch = 3;
 
 // Phase-1 steps whose coarse code loads are issued together (the loop over
 // the two groups of a chunk stays rolled: unrolled, it needs 151 VGPRs).
@@ -1100,6 +1101,16 @@ __device__ __forceinline__ bool fold_half(const float4 &r4, const float4 &b4, co
 // a refresh costs about as much as a quarter of the chunk's tests.
 constexpr uint32_t kThresholdEvery = 8;
 
+// Working memory of one wavefront in hash_piece_bounded (LDS): the tile's
+// 128 keyed minima and the (position, sample) items of one fine-test round,
+// at most kFineBatch steps x 8 samples per lane.
+constexpr uint32_t kItemCap = kWave * kFineBatch * 8;
+struct PieceLds {
+  unsigned long long key[128];
+  uint32_t item[kItemCap];
+  uint32_t count;
+};
+
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
                                                    const uint8_t *__restrict__ codes,
                                                    const uint8_t *__restrict__ coarse,
@@ -1107,11 +1118,15 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
                                                    uint32_t dim, uint32_t sp, uint32_t samples,
                                                    uint32_t tile, const MatrixDesc &M, uint32_t a,
                                                    uint32_t b, uint32_t row, uint32_t col0_in,
-                                                   bool fast_div, const float (&slot_best)[8]) {
+                                                   bool fast_div,
+                                                   const unsigned long long (&skey)[8]) {
+  __shared__ PieceLds piece_lds[kWavesPerBlock];
+  PieceLds &L = piece_lds[threadIdx.x >> 6];
   const uint32_t lane = lane_id();
   const uint32_t quarter = lane >> 4;
+  const uint32_t group = lane & 15u;
   const uint32_t col0 = uniform(col0_in);
-  const uint32_t k0 = tile * 128u + 8u * (lane & 15u);
+  const uint32_t k0 = tile * 128u + 8u * group;
   const bool lane_ok = k0 < samples;
   const uint32_t koff = lane_ok ? k0 : 0u;
   const uint32_t block_bytes = 12u * sp;  // the code table holds sp bytes per column
@@ -1140,17 +1155,23 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       __builtin_amdgcn_make_buffer_rsrc(cbase, (short)0, (int)0xffffffff, kRawBufferWord3);
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
-  float best[8], thr[8];
-  uint32_t pos[8];
+  // The tile's minima start as the row's keyed minima (the lanes of the
+  // first quarter hold them, 8 samples each; samples past the end: none) and
+  // take every item's key by an LDS atomic minimum, so all four quarters test
+  // against the minima of the whole piece so far.
+  if (quarter == 0u) {
 #pragma unroll
-  for (int v = 0; v < 8; v++) {
-    best[v] = FLT_MAX;
-    pos[v] = WMH_SENTINEL;
-    thr[v] = slot_best[v];
+    for (int v = 0; v < 8; v++) L.key[8u * group + v] = lane_ok ? skey[v] : kNoKey;
   }
+  if (lane == 0u) L.count = 0u;
   uint32_t tpair[4], tmax2;
-  code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
-  bool stale = false;  // the lane's minima moved below its thresholds
+  {
+    float thr[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) thr[v] = key_value(skey[v]);
+    code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
+  }
+  bool stale = false;  // the tile's minima moved below the lanes' thresholds
   uint32_t chunk_no = 0;
   // the next chunk's columns and weights are loaded while this one is tested
   uint32_t next_c = dim;
@@ -1217,89 +1238,101 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     uint32_t cmarked = ~cskip & (lane_ok ? (1u << live) - 1u : 0u);
     // Phase 1b, the fine test of the groups the coarse test kept, up to
     // kFineBatch per lane and round with their loads in flight together (a
-    // lane with fewer reads the stripe's first column, which stays in L2):
-    // bit m of `marked` when one of the lane's 8 samples of position
-    // cb + 4m + quarter is not skipped.
-    uint32_t marked = 0u;
+    // lane with fewer reads the stripe's first column, which stays in L2);
+    // every (position, sample) it does not skip becomes an item of the
+    // round's LDS list.  Phase 2 then evaluates the round's items exactly,
+    // one per lane (the formula of kernel.cu:103-106), and merges each into
+    // the tile's minimum of its sample with an LDS atomic minimum of the key
+    // (order(ln_a) << 32 | position): items are spread over the whole
+    // wavefront, so a lane evaluates one sample rather than the 8 of a group
+    // whose other 7 the codes already skip.
     while (__ballot(cmarked != 0u) != 0ull) {
-      uint32_t mk[kFineBatch], lpk[kFineBatch];
+      uint32_t mk[kFineBatch], lpk[kFineBatch], hk[kFineBatch];
       uint2 xk[kFineBatch];
 #pragma unroll
       for (int k = 0; k < kFineBatch; k++) {
-        const uint32_t has = cmarked != 0u ? 1u : 0u;
-        mk[k] = has ? (uint32_t)__builtin_ctz(cmarked) : 32u;
+        hk[k] = cmarked != 0u ? 1u : 0u;
+        mk[k] = hk[k] ? (uint32_t)__builtin_ctz(cmarked) : 0u;
         cmarked &= cmarked - 1u;
-        const uint32_t src = 4u * (mk[k] & 15u) + quarter;
-        const uint32_t ob_m = bperm_u32(ob, src) & (0u - has);  // arithmetic, not a select
+        const uint32_t src = 4u * mk[k] + quarter;
+        const uint32_t ob_m = bperm_u32(ob, src) & (0u - hk[k]);  // arithmetic, not a select
         lpk[k] = bperm_u32(lw_pair, src);
         xk[k] = __builtin_bit_cast(
             uint2, __builtin_amdgcn_raw_buffer_load_b64(rb, ob_m + koff, 0, 0));
       }
+      uint32_t kv[kFineBatch];  // bit v: sample v of step mk[k] kept
+      uint32_t nitems = 0u;
 #pragma unroll
       for (int k = 0; k < kFineBatch; k++) {
         // bit 15 of each half: that sample skipped (wmh_code_skips)
-        const uint32_t h = (code_pair<0>(xk[k].x) + tpair[0] + lpk[k]) &
-                           (code_pair<1>(xk[k].x) + tpair[1] + lpk[k]) &
-                           (code_pair<0>(xk[k].y) + tpair[2] + lpk[k]) &
-                           (code_pair<1>(xk[k].y) + tpair[3] + lpk[k]);
-        // (bit 32 of a 64-bit one: nothing for a lane without a step)
-        if ((h & 0x80008000u) != 0x80008000u) marked |= (uint32_t)(1ull << mk[k]);
+        const uint32_t h0 = code_pair<0>(xk[k].x) + tpair[0] + lpk[k];
+        const uint32_t h1 = code_pair<1>(xk[k].x) + tpair[1] + lpk[k];
+        const uint32_t h2 = code_pair<0>(xk[k].y) + tpair[2] + lpk[k];
+        const uint32_t h3 = code_pair<1>(xk[k].y) + tpair[3] + lpk[k];
+        const uint32_t skip = ((h0 >> 15) & 1u) | ((h0 >> 30) & 2u) | ((h1 >> 13) & 4u) |
+                              ((h1 >> 28) & 8u) | ((h2 >> 11) & 16u) | ((h2 >> 26) & 32u) |
+                              ((h3 >> 9) & 64u) | ((h3 >> 24) & 128u);
+        kv[k] = ~skip & 0xffu & (0u - hk[k]);
+        nitems += (uint32_t)__builtin_popcount(kv[k]);
       }
-    }
-    // phase 2: exact evaluation of the marked nonzeros, lowest position
-    // first (the division mode is picked per round: a round waits for its
-    // loads before its folds anyway)
-    for (;;) {
-      const bool has = marked != 0u;
-      if (__ballot(has) == 0ull) break;
-      const uint32_t m = has ? (uint32_t)__builtin_ctz(marked) : 0u;
-      marked &= marked - 1u;
-      const uint32_t src = 4u * m + quarter;
-      const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
-      const uint32_t ou = bperm_u32(o, src) + 4u * koff;
-      if (has) {
-        const float4 r0 = load4(rs, ou, 0u), b0 = load4(rs, ou, s_b), c0 = load4(rs, ou, s_c);
-        const float4 r1 = load4(rs, ou + 16u, 0u), b1 = load4(rs, ou + 16u, s_b),
-                     c1 = load4(rs, ou + 16u, s_c);
-        if (fast) {
-          stale |= fold_half<true, 0>(r0, b0, c0, lws, cb + src, best, pos);
-          stale |= fold_half<true, 1>(r1, b1, c1, lws, cb + src, best, pos);
-        } else {
-          stale |= fold_half<false, 0>(r0, b0, c0, lws, cb + src, best, pos);
-          stale |= fold_half<false, 1>(r1, b1, c1, lws, cb + src, best, pos);
+      if (__ballot(nitems != 0u) == 0ull) continue;
+      uint32_t at = 0u;
+      if (nitems != 0u) at = atomicAdd(&L.count, nitems);
+#pragma unroll
+      for (int k = 0; k < kFineBatch; k++) {
+        uint32_t bits = kv[k];
+        while (bits != 0u) {
+          const uint32_t v = (uint32_t)__builtin_ctz(bits);
+          bits &= bits - 1u;
+          L.item[at++] = mk[k] | (v << 4) | (lane << 7);
         }
       }
+      const uint32_t total = uniform(L.count);
+      if (lane == 0u) L.count = 0u;
+      stale = true;
+      for (uint32_t i0 = 0; i0 < total; i0 += kWave) {
+        const bool valid = i0 + lane < total;
+        const uint32_t it = valid ? L.item[i0 + lane] : 0u;
+        const uint32_t owner = it >> 7;
+        const uint32_t v = (it >> 4) & 7u;
+        const uint32_t src = 4u * (it & 15u) + (owner >> 4);
+        const uint32_t k = 8u * (owner & 15u) + v;  // the sample's place in the tile
+        const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
+        const uint32_t off = (bperm_u32(o, src) + 4u * (tile * 128u + k)) & (valid ? ~0u : 0u);
+        const float r = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0u, 0));
+        const float beta = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_b, 0));
+        const float lc = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_c, 0));
+        float ln_a;
+        {
+          WMH_NO_CONTRACT
+          const float t = __builtin_floorf((fast ? div_lw<true>(lws, r) : div_lw<false>(lws, r)) +
+                                           beta);             // kernel.cu:103
+          const float ln_y = (t - beta) * r;                  // kernel.cu:104
+          ln_a = (lc - ln_y) - r;                             // kernel.cu:105
+        }
+        // kernel.cu:106: below FLT_MAX, and the smallest (ln_a, position)
+        if (valid && ln_a < FLT_MAX)
+          atomicMin(&L.key[k], ((unsigned long long)order_key(ln_a) << 32) | (cb + src));
+      }
     }
-    if (++chunk_no % kThresholdEvery == 0u && __ballot(stale) != 0ull) {
+    if (++chunk_no % kThresholdEvery == 0u && stale) {
+      float thr[8];
 #pragma unroll
-      for (int v = 0; v < 8; v++) thr[v] = best[v] < thr[v] ? best[v] : thr[v];
+      for (int v = 0; v < 8; v++) thr[v] = key_value(L.key[8u * group + v]);
       code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
       stale = false;
     }
   }
-  // merge the quarters by (ln_a, position): lanes l, l ^ 16, l ^ 32, l ^ 48
+  // the tile's minima into the row's keyed minima (lane l: samples l and
+  // l + 64 of the tile; a slot only decreases)
+  unsigned long long *slot = reinterpret_cast<unsigned long long *>(M.output) +
+                             ((uint64_t)row * samples + tile * 128u);
 #pragma unroll
-  for (uint32_t d = 16u; d <= 32u; d <<= 1) {
-#pragma unroll
-    for (int v = 0; v < 8; v++) {
-      const float ob2 = __uint_as_float(bperm_u32(__float_as_uint(best[v]), lane ^ d));
-      const uint32_t op = bperm_u32(pos[v], lane ^ d);
-      const bool take = ob2 < best[v] || (ob2 == best[v] && op < pos[v]);
-      best[v] = take ? ob2 : best[v];
-      pos[v] = take ? op : pos[v];
-    }
-  }
-  if (quarter != 0u || !lane_ok) return true;
-  unsigned long long *slot =
-      reinterpret_cast<unsigned long long *>(M.output) + ((uint64_t)row * samples + k0);
-  unsigned long long seen[8];
-#pragma unroll
-  for (int v = 0; v < 8; v++) seen[v] = slot[v];
-#pragma unroll
-  for (int v = 0; v < 8; v++) {
-    if (pos[v] != WMH_SENTINEL) {
-      const unsigned long long key = ((unsigned long long)order_key(best[v]) << 32) | pos[v];
-      if (key < seen[v]) atomicMin(slot + v, key);
+  for (uint32_t h = 0; h < 2u; h++) {
+    const uint32_t k = lane + 64u * h;
+    if (tile * 128u + k < samples) {
+      const unsigned long long key = L.key[k];
+      if (key != kNoKey && key < slot[k]) atomicMin(slot + k, key);
     }
   }
   return true;
@@ -1319,8 +1352,12 @@ __device__ __forceinline__ uint32_t find_matrix(const MatrixDesc *__restrict__ m
   return uniform(lo);
 }
 
+// 4 waves per SIMD (128 VGPRs) at 1 and 2 samples per lane: left alone the
+// compiler takes 167 VGPRs for the 2-samples kernel and drops to 3 waves; the
+// 4-samples kernel (131 VGPRs) keeps 3.
 template <int V>
-__global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(V == 4 ? 3 : 4)))
+void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
     StripePlan stripes, uint32_t fast_div, const uint8_t *__restrict__ codes, float code_scale,
@@ -1433,20 +1470,19 @@ __global__ __launch_bounds__(kBlock) void wmh_persistent_kernel(
               // layout); once the earlier pieces left one for every sample
               // of the tile, the bound can skip most exact evaluations
               const uint32_t kq = tile * 128u + 8u * (lane & 15u);
-              float sbest[8];
+              unsigned long long skey[8];
               bool none = false;
               const unsigned long long *slot =
                   reinterpret_cast<const unsigned long long *>(M.output) +
                   ((uint64_t)row * samples + (kq < samples ? kq : 0u));
 #pragma unroll
               for (int v = 0; v < 8; v++) {
-                const unsigned long long key = slot[v];
-                none |= kq < samples && key == kNoKey;
-                sbest[v] = key_value(key);
+                skey[v] = slot[v];
+                none |= kq < samples && skey[v] == kNoKey;
               }
               if (__ballot(none) == 0ull &&
                   hash_piece_bounded(params, codes, coarse, code_scale, code_t0, dim, sp,
-                                     samples, tile, M, a, b, row, c0, fast_div != 0u, sbest))
+                                     samples, tile, M, a, b, row, c0, fast_div != 0u, skey))
                 continue;
             }
             hash_piece_pairs(params, dim, sp, samples, tile, M, a, b, row, c0, fast_div != 0u);
