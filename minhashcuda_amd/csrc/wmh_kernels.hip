@@ -1037,11 +1037,6 @@ __device__ __forceinline__ float key_value(unsigned long long key) {
   return __uint_as_float((h & 0x80000000u) ? (h & 0x7fffffffu) : ~h);
 }
 
-// Fine tests per lane and round whose code loads are issued together (1, 2,
-// 3 and 4 measured 182.9 / 175.1 / 167.0 / 176.4 ms on the uneven batch:
-// fewer wait for more round trips, more test for idle lanes and take more
-// LDS for the round's items).
-constexpr int kFineBatch = 3;
 
 // Phase-1 steps whose coarse code loads are issued together (the loop over
 // the two groups of a chunk stays rolled: unrolled, it needs 151 VGPRs).
@@ -1102,13 +1097,16 @@ __device__ __forceinline__ bool fold_half(const float4 &r4, const float4 &b4, co
 constexpr uint32_t kThresholdEvery = 8;
 
 // Working memory of one wavefront in hash_piece_bounded (LDS): the tile's
-// 128 keyed minima and the (position, sample) items of one fine-test round,
-// at most kFineBatch steps x 8 samples per lane.
-constexpr uint32_t kItemCap = kWave * kFineBatch * 8;
+// 128 keyed minima, every lane's packed code thresholds, the chunk's
+// (position, lane) pairs that passed the coarse test (at most 16 per lane)
+// and the (position, sample) items of one fine-test pass (at most 8 per
+// lane).
 struct PieceLds {
   unsigned long long key[128];
-  uint32_t item[kItemCap];
-  uint32_t count;
+  uint4 tpair[kWave];
+  uint32_t pair[kWave * 16];
+  uint32_t item[kWave * 8];
+  uint32_t npair, nitem;
 };
 
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
@@ -1163,13 +1161,17 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
 #pragma unroll
     for (int v = 0; v < 8; v++) L.key[8u * group + v] = lane_ok ? skey[v] : kNoKey;
   }
-  if (lane == 0u) L.count = 0u;
+  if (lane == 0u) {
+    L.npair = 0u;
+    L.nitem = 0u;
+  }
   uint32_t tpair[4], tmax2;
   {
     float thr[8];
 #pragma unroll
     for (int v = 0; v < 8; v++) thr[v] = key_value(skey[v]);
     code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
+    L.tpair[lane] = make_uint4(tpair[0], tpair[1], tpair[2], tpair[3]);
   }
   bool stale = false;  // the tile's minima moved below the lanes' thresholds
   uint32_t chunk_no = 0;
@@ -1236,83 +1238,87 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     }
     const uint32_t live = n > quarter ? (n - quarter + 3u) >> 2 : 0u;  // positions of this quarter
     uint32_t cmarked = ~cskip & (lane_ok ? (1u << live) - 1u : 0u);
-    // Phase 1b, the fine test of the groups the coarse test kept, up to
-    // kFineBatch per lane and round with their loads in flight together (a
-    // lane with fewer reads the stripe's first column, which stays in L2);
-    // every (position, sample) it does not skip becomes an item of the
-    // round's LDS list.  Phase 2 then evaluates the round's items exactly,
-    // one per lane (the formula of kernel.cu:103-106), and merges each into
-    // the tile's minimum of its sample with an LDS atomic minimum of the key
-    // (order(ln_a) << 32 | position): items are spread over the whole
-    // wavefront, so a lane evaluates one sample rather than the 8 of a group
-    // whose other 7 the codes already skip.
-    while (__ballot(cmarked != 0u) != 0ull) {
-      uint32_t mk[kFineBatch], lpk[kFineBatch], hk[kFineBatch];
-      uint2 xk[kFineBatch];
-#pragma unroll
-      for (int k = 0; k < kFineBatch; k++) {
-        hk[k] = cmarked != 0u ? 1u : 0u;
-        mk[k] = hk[k] ? (uint32_t)__builtin_ctz(cmarked) : 0u;
-        cmarked &= cmarked - 1u;
-        const uint32_t src = 4u * mk[k] + quarter;
-        const uint32_t ob_m = bperm_u32(ob, src) & (0u - hk[k]);  // arithmetic, not a select
-        lpk[k] = bperm_u32(lw_pair, src);
-        xk[k] = __builtin_bit_cast(
-            uint2, __builtin_amdgcn_raw_buffer_load_b64(rb, ob_m + koff, 0, 0));
-      }
-      uint32_t kv[kFineBatch];  // bit v: sample v of step mk[k] kept
-      uint32_t nitems = 0u;
-#pragma unroll
-      for (int k = 0; k < kFineBatch; k++) {
-        // bit 15 of each half: that sample skipped (wmh_code_skips)
-        const uint32_t h0 = code_pair<0>(xk[k].x) + tpair[0] + lpk[k];
-        const uint32_t h1 = code_pair<1>(xk[k].x) + tpair[1] + lpk[k];
-        const uint32_t h2 = code_pair<0>(xk[k].y) + tpair[2] + lpk[k];
-        const uint32_t h3 = code_pair<1>(xk[k].y) + tpair[3] + lpk[k];
-        const uint32_t skip = ((h0 >> 15) & 1u) | ((h0 >> 30) & 2u) | ((h1 >> 13) & 4u) |
-                              ((h1 >> 28) & 8u) | ((h2 >> 11) & 16u) | ((h2 >> 26) & 32u) |
-                              ((h3 >> 9) & 64u) | ((h3 >> 24) & 128u);
-        kv[k] = ~skip & 0xffu & (0u - hk[k]);
-        nitems += (uint32_t)__builtin_popcount(kv[k]);
-      }
-      if (__ballot(nitems != 0u) == 0ull) continue;
+    // Phase 1b, the fine test of the (position, lane) pairs the coarse test
+    // kept (a few percent), spread over the wavefront: the pairs go to an LDS
+    // list and each lane tests one of them against the 8 codes and the owning
+    // lane's thresholds.  Every (position, sample) it does not skip becomes
+    // an item.  Phase 2 evaluates the items, one per lane (the formula of
+    // kernel.cu:103-106 for one sample), and merges each into the tile's
+    // minimum of its sample with an LDS atomic minimum of the key
+    // (order(ln_a) << 32 | position).
+    if (__ballot(cmarked != 0u) != 0ull) {
+      const uint32_t np = (uint32_t)__builtin_popcount(cmarked);
       uint32_t at = 0u;
-      if (nitems != 0u) at = atomicAdd(&L.count, nitems);
-#pragma unroll
-      for (int k = 0; k < kFineBatch; k++) {
-        uint32_t bits = kv[k];
-        while (bits != 0u) {
-          const uint32_t v = (uint32_t)__builtin_ctz(bits);
-          bits &= bits - 1u;
-          L.item[at++] = mk[k] | (v << 4) | (lane << 7);
-        }
+      if (np != 0u) at = atomicAdd(&L.npair, np);
+      while (cmarked != 0u) {
+        const uint32_t m = (uint32_t)__builtin_ctz(cmarked);
+        cmarked &= cmarked - 1u;
+        L.pair[at++] = m | (lane << 4);
       }
-      const uint32_t total = uniform(L.count);
-      if (lane == 0u) L.count = 0u;
+      const uint32_t pairs = uniform(L.npair);
+      if (lane == 0u) L.npair = 0u;
       stale = true;
-      for (uint32_t i0 = 0; i0 < total; i0 += kWave) {
-        const bool valid = i0 + lane < total;
-        const uint32_t it = valid ? L.item[i0 + lane] : 0u;
-        const uint32_t owner = it >> 7;
-        const uint32_t v = (it >> 4) & 7u;
-        const uint32_t src = 4u * (it & 15u) + (owner >> 4);
-        const uint32_t k = 8u * (owner & 15u) + v;  // the sample's place in the tile
-        const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), src));
-        const uint32_t off = (bperm_u32(o, src) + 4u * (tile * 128u + k)) & (valid ? ~0u : 0u);
-        const float r = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0u, 0));
-        const float beta = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_b, 0));
-        const float lc = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_c, 0));
-        float ln_a;
-        {
-          WMH_NO_CONTRACT
-          const float t = __builtin_floorf((fast ? div_lw<true>(lws, r) : div_lw<false>(lws, r)) +
-                                           beta);             // kernel.cu:103
-          const float ln_y = (t - beta) * r;                  // kernel.cu:104
-          ln_a = (lc - ln_y) - r;                             // kernel.cu:105
+      for (uint32_t p0 = 0; p0 < pairs; p0 += kWave) {
+        const bool valid = p0 + lane < pairs;
+        const uint32_t pr = valid ? L.pair[p0 + lane] : 0u;
+        const uint32_t owner = pr >> 4;
+        const uint32_t src = 4u * (pr & 15u) + (owner >> 4);
+        const uint32_t ob_m = bperm_u32(ob, src);
+        const uint32_t lp = bperm_u32(lw_pair, src);
+        const uint4 tp = L.tpair[owner];
+        const uint32_t ko = tile * 128u + 8u * (owner & 15u);  // the owner's samples
+        const uint2 x = __builtin_bit_cast(
+            uint2, __builtin_amdgcn_raw_buffer_load_b64(rb, (ob_m + ko) & (valid ? ~0u : 0u),
+                                                         0, 0));
+        // bit 15 of each half: that sample skipped (wmh_code_skips)
+        const uint32_t h0 = code_pair<0>(x.x) + tp.x + lp;
+        const uint32_t h1 = code_pair<1>(x.x) + tp.y + lp;
+        const uint32_t h2 = code_pair<0>(x.y) + tp.z + lp;
+        const uint32_t h3 = code_pair<1>(x.y) + tp.w + lp;
+        // the 8 skip bits (bit 7 of bytes 1 and 3 of each sum) gathered into
+        // the low byte: two bytes per sample pair, then a multiply that adds
+        // the four bits of a word into its top byte
+        const uint32_t w0 = (__builtin_amdgcn_perm(h1, h0, 0x07050301u) >> 7) & 0x01010101u;
+        const uint32_t w1 = (__builtin_amdgcn_perm(h3, h2, 0x07050301u) >> 7) & 0x01010101u;
+        const uint32_t skip = ((w0 * 0x01020408u) >> 24) | (((w1 * 0x01020408u) >> 24) << 4);
+        uint32_t kv = valid ? ~skip & 0xffu : 0u;  // bit v: sample v kept
+        const uint32_t ni = (uint32_t)__builtin_popcount(kv);
+        uint32_t ia = 0u;
+        if (ni != 0u) ia = atomicAdd(&L.nitem, ni);
+        while (kv != 0u) {
+          const uint32_t v = (uint32_t)__builtin_ctz(kv);
+          kv &= kv - 1u;
+          L.item[ia++] = (pr & 15u) | (v << 4) | (owner << 7);
         }
-        // kernel.cu:106: below FLT_MAX, and the smallest (ln_a, position)
-        if (valid && ln_a < FLT_MAX)
-          atomicMin(&L.key[k], ((unsigned long long)order_key(ln_a) << 32) | (cb + src));
+        const uint32_t total = uniform(L.nitem);
+        if (lane == 0u) L.nitem = 0u;
+        for (uint32_t i0 = 0; i0 < total; i0 += kWave) {
+          const bool ivalid = i0 + lane < total;
+          const uint32_t it = ivalid ? L.item[i0 + lane] : 0u;
+          const uint32_t io = it >> 7;
+          const uint32_t isrc = 4u * (it & 15u) + (io >> 4);
+          const uint32_t k = 8u * (io & 15u) + ((it >> 4) & 7u);  // the sample's place in the tile
+          const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), isrc));
+          const uint32_t off =
+              (bperm_u32(o, isrc) + 4u * (tile * 128u + k)) & (ivalid ? ~0u : 0u);
+          const float r =
+              __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0u, 0));
+          const float beta =
+              __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_b, 0));
+          const float lc =
+              __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_c, 0));
+          float ln_a;
+          {
+            WMH_NO_CONTRACT
+            const float t = __builtin_floorf(
+                (fast ? div_lw<true>(lws, r) : div_lw<false>(lws, r)) + beta);  // kernel.cu:103
+            const float ln_y = (t - beta) * r;                                   // kernel.cu:104
+            ln_a = (lc - ln_y) - r;                                              // kernel.cu:105
+          }
+          // kernel.cu:106: below FLT_MAX, and the smallest (ln_a, position)
+          if (ivalid && ln_a < FLT_MAX)
+            atomicMin(&L.key[k], ((unsigned long long)order_key(ln_a) << 32) | (cb + isrc));
+        }
       }
     }
     if (++chunk_no % kThresholdEvery == 0u && stale) {
@@ -1320,6 +1326,7 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
 #pragma unroll
       for (int v = 0; v < 8; v++) thr[v] = key_value(L.key[8u * group + v]);
       code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
+      L.tpair[lane] = make_uint4(tpair[0], tpair[1], tpair[2], tpair[3]);
       stale = false;
     }
   }
