@@ -474,7 +474,7 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
       gen->bounds = nullptr;
       INFO("no memory for the bound table: hashing without the skip bound\n");
     } else if (gen->sp % 8u == 0u) {
-      gen->coarse = gen->bounds + (size_t)dim * gen->sp;
+      gen->coarse = gen->bounds + wmh::coarse_offset(dim, samples);
     }
   }
   if (deferred) {

@@ -137,8 +137,9 @@ hipError_t launch_params_range(const float *params, uint32_t dim, uint32_t sampl
 // The byte-coded bound table of the skip bound (wmh_bound.h): `codes`
 // [dim][sp] bytes, and the scale 1 / delta and offset t0 = 2049 - lo / delta
 // of its thresholds.  codes == null: the skip bound is not used.  `coarse`
-// [dim][sp / 8] holds the smallest code of every 8-sample group (null unless
-// sp % 8 == 0); the bounded stripe pieces test it before the 8 codes.
+// [dim][coarse_row_bytes(sp)] holds the smallest code of every 8-sample group
+// (null unless sp % 8 == 0); the bounded stripe pieces test it before the 8
+// codes.
 struct CodeTable {
   const uint8_t *codes;
   float scale;
@@ -146,10 +147,22 @@ struct CodeTable {
   const uint8_t *coarse = nullptr;
 };
 
-// Bytes of the bound table with its coarse codes for dim x samples.
+// Bytes of a column's coarse codes: one per 8-sample group, padded to a
+// multiple of 16 so that the 16 groups of a 128-sample tile are one aligned
+// 16-byte load.
+__host__ __device__ inline uint32_t coarse_row_bytes(uint32_t sp) {
+  return ((sp / 8u) + 15u) & ~15u;
+}
+
+// Offset of the coarse codes in the bound table's allocation (after the
+// codes, 256-byte aligned) and the allocation's bytes for dim x samples.
+inline uint64_t coarse_offset(uint32_t dim, uint32_t samples) {
+  return ((uint64_t)dim * padded_samples(samples) + 255u) & ~255ull;
+}
 inline uint64_t code_table_bytes(uint32_t dim, uint32_t samples) {
-  const uint64_t sp = padded_samples(samples);
-  return (uint64_t)dim * sp + (sp % 8u == 0u ? (uint64_t)dim * (sp / 8u) : 0u);
+  const uint32_t sp = padded_samples(samples);
+  return sp % 8u == 0u ? coarse_offset(dim, samples) + (uint64_t)dim * coarse_row_bytes(sp)
+                       : (uint64_t)dim * sp;
 }
 
 // Smallest and largest bound entry (wmh_bound_entry) over the real samples
@@ -161,7 +174,8 @@ hipError_t launch_bound_range(const float *params, uint32_t dim, uint32_t sample
 
 // Fills codes [dim][sp] with wmh_bound_code of every entry for the given lo
 // and delta (padded samples: 255), then, when coarse is not null (sp % 8 ==
-// 0), coarse [dim][sp / 8] with the smallest of every 8 consecutive codes.
+// 0), coarse [dim][coarse_row_bytes(sp)] with the smallest of every 8
+// consecutive codes (padding bytes: 255).
 hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t samples, float lo,
                               float delta, uint8_t *codes, uint8_t *coarse, hipStream_t stream);
 

@@ -240,23 +240,28 @@ __global__ __launch_bounds__(kBlock) void build_codes_kernel(const float *__rest
   }
 }
 
-// coarse[i] = the smallest of codes[8i .. 8i + 7]: per column, the smallest
-// code of each 8-sample group (a column's sp codes start at a multiple of 8
-// when sp % 8 == 0).  A group whose smallest code is skipped against the
-// group's largest threshold T' has every code skipped against its own
-// threshold: the skip test (wmh_code_skips) only turns on as c grows or T'
-// falls.
+// coarse[j][g] = the smallest of codes[j][8g .. 8g + 7]: per column, the
+// smallest code of each 8-sample group, in rows of rb bytes (padding: 255).
+// A group whose smallest code is skipped against the group's largest
+// threshold T' has every code skipped against its own threshold: the skip
+// test (wmh_code_skips) only turns on as c grows or T' falls.
 __global__ __launch_bounds__(kBlock) void build_coarse_kernel(const uint8_t *__restrict__ codes,
-                                                              uint64_t groups,
+                                                              uint32_t dim, uint32_t sp,
+                                                              uint32_t rb,
                                                               uint8_t *__restrict__ coarse) {
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < groups;
+  const uint64_t total = (uint64_t)dim * rb;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
        i += (uint64_t)gridDim.x * kBlock) {
-    const uint2 w = *reinterpret_cast<const uint2 *>(codes + 8u * i);
+    const uint64_t j = i / rb;
+    const uint32_t g = (uint32_t)(i - j * rb);
     uint32_t m = 255u;
+    if (g < sp / 8u) {
+      const uint2 w = *reinterpret_cast<const uint2 *>(codes + j * sp + 8u * g);
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-      m = min(m, (w.x >> (8 * b)) & 0xffu);
-      m = min(m, (w.y >> (8 * b)) & 0xffu);
+      for (int b = 0; b < 4; b++) {
+        m = min(m, (w.x >> (8 * b)) & 0xffu);
+        m = min(m, (w.y >> (8 * b)) & 0xffu);
+      }
     }
     coarse[i] = (uint8_t)m;
   }
@@ -1097,13 +1102,13 @@ __device__ __forceinline__ bool fold_half(const float4 &r4, const float4 &b4, co
 constexpr uint32_t kThresholdEvery = 8;
 
 // Working memory of one wavefront in hash_piece_bounded (LDS): the tile's
-// 128 keyed minima, every lane's packed code thresholds, the chunk's
-// (position, lane) pairs that passed the coarse test (at most 16 per lane)
-// and the (position, sample) items of one fine-test pass (at most 8 per
-// lane).
+// 128 keyed minima, the packed code thresholds of its samples (two per
+// word), the chunk's (nonzero, group) pairs that passed the coarse test (at
+// most 16 per lane) and the (nonzero, sample) items of one fine-test pass
+// (at most 8 per lane).
 struct PieceLds {
   unsigned long long key[128];
-  uint4 tpair[kWave];
+  uint32_t tq[64];
   uint32_t pair[kWave * 16];
   uint32_t item[kWave * 8];
   uint32_t npair, nitem;
@@ -1121,12 +1126,8 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
   __shared__ PieceLds piece_lds[kWavesPerBlock];
   PieceLds &L = piece_lds[threadIdx.x >> 6];
   const uint32_t lane = lane_id();
-  const uint32_t quarter = lane >> 4;
-  const uint32_t group = lane & 15u;
   const uint32_t col0 = uniform(col0_in);
-  const uint32_t k0 = tile * 128u + 8u * group;
-  const bool lane_ok = k0 < samples;
-  const uint32_t koff = lane_ok ? k0 : 0u;
+  const uint32_t t0k = tile * 128u;  // the tile's first sample
   const uint32_t block_bytes = 12u * sp;  // the code table holds sp bytes per column
   const uint32_t s_b = uniform(sp * 4u), s_c = uniform(sp * 8u);  // beta, ln_c arrays
   // column offsets c - col0 up to `span` keep every lane's bytes below 2^32
@@ -1142,10 +1143,9 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       __builtin_amdgcn_make_buffer_rsrc(pbase, (short)0, (int)0xffffffff, kRawBufferWord3);
   const __amdgpu_buffer_rsrc_t rb =
       __builtin_amdgcn_make_buffer_rsrc(bbase, (short)0, (int)0xffffffff, kRawBufferWord3);
-  // coarse codes: sp / 8 bytes per column, byte g of a column for the lane's
-  // 8-sample group
-  const uint32_t spc = sp >> 3;
-  const uint32_t gbyte = koff >> 3;
+  // coarse codes: coarse_row_bytes(sp) per column, the tile's 16 groups as
+  // one aligned 16-byte load
+  const uint32_t spc = uniform(coarse_row_bytes(sp));
   const uint64_t cbb = reinterpret_cast<uint64_t>(coarse) + (uint64_t)col0 * spc;
   char *cbase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(cbb >> 32)) << 32) |
                                          uniform((uint32_t)cbb));
@@ -1153,27 +1153,38 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       __builtin_amdgcn_make_buffer_rsrc(cbase, (short)0, (int)0xffffffff, kRawBufferWord3);
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
+  // groups of the tile that hold samples (samples % 8 == 0: whole groups)
+  const uint32_t ngroups = uniform(min(16u, (samples - t0k) / 8u));
+  const uint32_t group_mask = ngroups >= 16u ? 0xffffu : (1u << ngroups) - 1u;
   // The tile's minima start as the row's keyed minima (the lanes of the
   // first quarter hold them, 8 samples each; samples past the end: none) and
-  // take every item's key by an LDS atomic minimum, so all four quarters test
-  // against the minima of the whole piece so far.
-  if (quarter == 0u) {
+  // take every item's key by an LDS atomic minimum.
+  if (lane < 16u) {
 #pragma unroll
-    for (int v = 0; v < 8; v++) L.key[8u * group + v] = lane_ok ? skey[v] : kNoKey;
+    for (int v = 0; v < 8; v++) L.key[8u * lane + v] = lane < ngroups ? skey[v] : kNoKey;
   }
   if (lane == 0u) {
     L.npair = 0u;
     L.nitem = 0u;
   }
-  uint32_t tpair[4], tmax2;
-  {
-    float thr[8];
+  // Code thresholds from the tile's minima: tq[l] packs 0x4800 - T' of
+  // samples 2l and 2l + 1 (the fine test), tc[i] that of the largest T' of
+  // groups 2i and 2i + 1 (the coarse test, wave-uniform).
+  uint32_t tc[8];
+  auto refresh = [&]() {
+    const uint32_t ta = wmh_code_threshold(key_value(L.key[2u * lane]), code_scale, code_t0);
+    const uint32_t tb = wmh_code_threshold(key_value(L.key[2u * lane + 1u]), code_scale, code_t0);
+    L.tq[lane] = (0x4800u - ta) | ((0x4800u - tb) << 16);
+    uint32_t m = max(ta, tb);
+    m = max(m, (uint32_t)__shfl_xor((int)m, 1));
+    m = max(m, (uint32_t)__shfl_xor((int)m, 2));  // lanes 4g .. 4g + 3: group g's largest
+    const uint32_t cg = 0x4800u - m;
 #pragma unroll
-    for (int v = 0; v < 8; v++) thr[v] = key_value(skey[v]);
-    code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
-    L.tpair[lane] = make_uint4(tpair[0], tpair[1], tpair[2], tpair[3]);
-  }
-  bool stale = false;  // the tile's minima moved below the lanes' thresholds
+    for (int i = 0; i < 8; i++)
+      tc[i] = readlane_u32(cg, 8u * i) | (readlane_u32(cg, 8u * i + 4u) << 16);
+  };
+  refresh();
+  bool stale = false;  // the tile's minima moved below the thresholds
   uint32_t chunk_no = 0;
   // the next chunk's columns and weights are loaded while this one is tested
   uint32_t next_c = dim;
@@ -1191,69 +1202,66 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       next_c = cols[cb + kWave + lane];
       next_w = weights[cb + kWave + lane];
     }
-    // lanes without a selectable nonzero gather from a valid block (the
-    // stripe's first, or column 0 for a stripe past dim) and are never
-    // selected (lw = NaN)
+    // lane l holds the nonzero at position cb + l; lanes without a
+    // selectable one point at a valid block (the stripe's first, or column 0
+    // for a stripe past dim) and list no pairs
+    const bool own = lane < n && cc < dim;
     uint32_t c = col0 < dim ? col0 : 0u;
     float lw = __builtin_nanf("");
     float lwb = -__builtin_inff();
-    if (lane < n && cc < dim) {
+    if (own) {
       c = cc;
       lw = wmh_logf(wv);  // once per nonzero (kernel.cu:95)
       lwb = wmh_bound_lw(lw);
     }
-    // every gather goes through the two resources with 32-bit offsets; a
-    // piece holding a column too far from the stripe for that (an unsorted
-    // row in a table of 4 GiB or more, or a stripe starting at dim, whose
-    // lanes without a column gather from column 0) is left to
-    // hash_piece_pairs, before anything of it is written
+    // every gather goes through the resources with 32-bit offsets; a piece
+    // holding a column too far from the stripe for that (an unsorted row in
+    // a table of 4 GiB or more, or a stripe starting at dim, whose lanes
+    // without a column point at column 0) is left to hash_piece_pairs,
+    // before anything of it is written
     if (__ballot(c - col0 > span) != 0ull) return false;
     const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
     const uint32_t rel = c - col0;
-    const uint32_t o = rel * block_bytes, ob = rel * sp, oc = rel * spc;
+    const uint32_t o = rel * block_bytes, ob = rel * sp;
     const uint32_t lw_term = wmh_code_lw(lwb, code_scale);
     const uint32_t lw_pair = lw_term | (lw_term << 16);
-    // Phase 1a, the coarse test: bit m of cskip when the group's smallest
-    // code, tested against the lane's largest threshold, is skipped -- then
-    // all 8 of its samples are.  Steps m and m + 4 of a group of 8 share one
-    // 32-bit add (16-bit halves, as in the fine test).
-    const uint32_t steps = uniform((n + 3u) >> 2);
-    uint32_t cskip = 0u;
-    for (uint32_t m0 = 0; m0 < steps; m0 += kBoundGroup) {
-      uint32_t x[kBoundGroup];
+    // Phase 1a, the coarse test of the lane's own nonzero for the tile's 16
+    // groups: the smallest code of each group against the group's largest
+    // threshold; a group it skips has all of its samples skipped.  Two groups
+    // per 32-bit add (16-bit halves); bit g of `kept` for group g.
+    const uint4 cw = __builtin_bit_cast(
+        uint4, __builtin_amdgcn_raw_buffer_load_b128(rc, rel * spc + 16u * tile, 0, 0));
+    const uint32_t h[8] = {code_pair<0>(cw.x) + tc[0] + lw_pair, code_pair<1>(cw.x) + tc[1] + lw_pair,
+                           code_pair<0>(cw.y) + tc[2] + lw_pair, code_pair<1>(cw.y) + tc[3] + lw_pair,
+                           code_pair<0>(cw.z) + tc[4] + lw_pair, code_pair<1>(cw.z) + tc[5] + lw_pair,
+                           code_pair<0>(cw.w) + tc[6] + lw_pair, code_pair<1>(cw.w) + tc[7] + lw_pair};
+    // the skip bits (bit 7 of bytes 1 and 3 of each sum) gathered four at a
+    // time: the bytes side by side, then a multiply that adds the four bits
+    // of a word into its top byte
+    uint32_t skip = 0u;
 #pragma unroll
-      for (int u = 0; u < kBoundGroup; u++) {
-        const uint32_t src = 4u * (m0 + u) + quarter;  // lanes past n read valid data
-        x[u] = __builtin_amdgcn_raw_buffer_load_b8(rc, bperm_u32(oc, src) + gbyte, 0, 0);
-      }
-#pragma unroll
-      for (int u = 0; u < kBoundGroup / 2; u++) {
-        const uint32_t lp = bperm_u32(lw_term, 4u * (m0 + u) + quarter) |
-                            (bperm_u32(lw_term, 4u * (m0 + u + 4u) + quarter) << 16);
-        // bit 15 / 31: step m0 + u / m0 + u + 4 skipped (wmh_code_skips)
-        const uint32_t h = __builtin_amdgcn_perm(x[u + 4], x[u], 0x0c040c00u) + tmax2 + lp;
-        cskip |= ((h >> (15u - (m0 + u))) & (1u << (m0 + u))) |
-                 ((h >> (27u - (m0 + u))) & (1u << (m0 + u + 4u)));
-      }
+    for (int i = 0; i < 4; i++) {
+      const uint32_t w = (__builtin_amdgcn_perm(h[2 * i + 1], h[2 * i], 0x07050301u) >> 7) & 0x01010101u;
+      skip |= ((w * 0x01020408u) >> 24) << (4 * i);
     }
-    const uint32_t live = n > quarter ? (n - quarter + 3u) >> 2 : 0u;  // positions of this quarter
-    uint32_t cmarked = ~cskip & (lane_ok ? (1u << live) - 1u : 0u);
-    // Phase 1b, the fine test of the (position, lane) pairs the coarse test
-    // kept (a few percent), spread over the wavefront: the pairs go to an LDS
-    // list and each lane tests one of them against the 8 codes and the owning
-    // lane's thresholds.  Every (position, sample) it does not skip becomes
-    // an item.  Phase 2 evaluates the items, one per lane (the formula of
+    uint32_t kept = own ? ~skip & group_mask : 0u;
+    // Phase 1b, the fine test of the (nonzero, group) pairs the coarse test
+    // kept (a few percent), spread over the wavefront through an LDS list:
+    // each lane tests one pair against its 8 codes and the 8 samples'
+    // thresholds, and every (nonzero, sample) it does not skip becomes an
+    // item.  Phase 2 evaluates the items, one per lane (the formula of
     // kernel.cu:103-106 for one sample), and merges each into the tile's
     // minimum of its sample with an LDS atomic minimum of the key
-    // (order(ln_a) << 32 | position).
-    if (__ballot(cmarked != 0u) != 0ull) {
-      const uint32_t np = (uint32_t)__builtin_popcount(cmarked);
+    // (order(ln_a) << 32 | position), which keeps the earliest of equal
+    // values.
+    if (__ballot(kept != 0u) != 0ull) {
+      const uint32_t np = (uint32_t)__builtin_popcount(kept);
       uint32_t at = 0u;
       if (np != 0u) at = atomicAdd(&L.npair, np);
-      while (cmarked != 0u) {
-        const uint32_t m = (uint32_t)__builtin_ctz(cmarked);
-        cmarked &= cmarked - 1u;
-        L.pair[at++] = m | (lane << 4);
+      while (kept != 0u) {
+        const uint32_t g = (uint32_t)__builtin_ctz(kept);
+        kept &= kept - 1u;
+        L.pair[at++] = lane | (g << 6);
       }
       const uint32_t pairs = uniform(L.npair);
       if (lane == 0u) L.npair = 0u;
@@ -1261,46 +1269,39 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       for (uint32_t p0 = 0; p0 < pairs; p0 += kWave) {
         const bool valid = p0 + lane < pairs;
         const uint32_t pr = valid ? L.pair[p0 + lane] : 0u;
-        const uint32_t owner = pr >> 4;
-        const uint32_t src = 4u * (pr & 15u) + (owner >> 4);
+        const uint32_t src = pr & 63u, g = pr >> 6;
         const uint32_t ob_m = bperm_u32(ob, src);
         const uint32_t lp = bperm_u32(lw_pair, src);
-        const uint4 tp = L.tpair[owner];
-        const uint32_t ko = tile * 128u + 8u * (owner & 15u);  // the owner's samples
+        const uint4 tp = *reinterpret_cast<const uint4 *>(&L.tq[4u * g]);
         const uint2 x = __builtin_bit_cast(
-            uint2, __builtin_amdgcn_raw_buffer_load_b64(rb, (ob_m + ko) & (valid ? ~0u : 0u),
-                                                         0, 0));
+            uint2, __builtin_amdgcn_raw_buffer_load_b64(
+                       rb, (ob_m + t0k + 8u * g) & (valid ? ~0u : 0u), 0, 0));
         // bit 15 of each half: that sample skipped (wmh_code_skips)
-        const uint32_t h0 = code_pair<0>(x.x) + tp.x + lp;
-        const uint32_t h1 = code_pair<1>(x.x) + tp.y + lp;
-        const uint32_t h2 = code_pair<0>(x.y) + tp.z + lp;
-        const uint32_t h3 = code_pair<1>(x.y) + tp.w + lp;
-        // the 8 skip bits (bit 7 of bytes 1 and 3 of each sum) gathered into
-        // the low byte: two bytes per sample pair, then a multiply that adds
-        // the four bits of a word into its top byte
-        const uint32_t w0 = (__builtin_amdgcn_perm(h1, h0, 0x07050301u) >> 7) & 0x01010101u;
-        const uint32_t w1 = (__builtin_amdgcn_perm(h3, h2, 0x07050301u) >> 7) & 0x01010101u;
-        const uint32_t skip = ((w0 * 0x01020408u) >> 24) | (((w1 * 0x01020408u) >> 24) << 4);
-        uint32_t kv = valid ? ~skip & 0xffu : 0u;  // bit v: sample v kept
+        const uint32_t f0 = code_pair<0>(x.x) + tp.x + lp;
+        const uint32_t f1 = code_pair<1>(x.x) + tp.y + lp;
+        const uint32_t f2 = code_pair<0>(x.y) + tp.z + lp;
+        const uint32_t f3 = code_pair<1>(x.y) + tp.w + lp;
+        const uint32_t w0 = (__builtin_amdgcn_perm(f1, f0, 0x07050301u) >> 7) & 0x01010101u;
+        const uint32_t w1 = (__builtin_amdgcn_perm(f3, f2, 0x07050301u) >> 7) & 0x01010101u;
+        const uint32_t fskip = ((w0 * 0x01020408u) >> 24) | (((w1 * 0x01020408u) >> 24) << 4);
+        uint32_t kv = valid ? ~fskip & 0xffu : 0u;  // bit v: sample 8g + v kept
         const uint32_t ni = (uint32_t)__builtin_popcount(kv);
         uint32_t ia = 0u;
         if (ni != 0u) ia = atomicAdd(&L.nitem, ni);
         while (kv != 0u) {
           const uint32_t v = (uint32_t)__builtin_ctz(kv);
           kv &= kv - 1u;
-          L.item[ia++] = (pr & 15u) | (v << 4) | (owner << 7);
+          L.item[ia++] = src | ((8u * g + v) << 6);
         }
         const uint32_t total = uniform(L.nitem);
         if (lane == 0u) L.nitem = 0u;
         for (uint32_t i0 = 0; i0 < total; i0 += kWave) {
           const bool ivalid = i0 + lane < total;
           const uint32_t it = ivalid ? L.item[i0 + lane] : 0u;
-          const uint32_t io = it >> 7;
-          const uint32_t isrc = 4u * (it & 15u) + (io >> 4);
-          const uint32_t k = 8u * (io & 15u) + ((it >> 4) & 7u);  // the sample's place in the tile
+          const uint32_t isrc = it & 63u;
+          const uint32_t k = it >> 6;  // the sample's place in the tile
           const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), isrc));
-          const uint32_t off =
-              (bperm_u32(o, isrc) + 4u * (tile * 128u + k)) & (ivalid ? ~0u : 0u);
+          const uint32_t off = (bperm_u32(o, isrc) + 4u * (t0k + k)) & (ivalid ? ~0u : 0u);
           const float r =
               __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0u, 0));
           const float beta =
@@ -1322,22 +1323,18 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
       }
     }
     if (++chunk_no % kThresholdEvery == 0u && stale) {
-      float thr[8];
-#pragma unroll
-      for (int v = 0; v < 8; v++) thr[v] = key_value(L.key[8u * group + v]);
-      code_thresholds(thr, code_scale, code_t0, tpair, tmax2);
-      L.tpair[lane] = make_uint4(tpair[0], tpair[1], tpair[2], tpair[3]);
+      refresh();
       stale = false;
     }
   }
   // the tile's minima into the row's keyed minima (lane l: samples l and
   // l + 64 of the tile; a slot only decreases)
-  unsigned long long *slot = reinterpret_cast<unsigned long long *>(M.output) +
-                             ((uint64_t)row * samples + tile * 128u);
+  unsigned long long *slot =
+      reinterpret_cast<unsigned long long *>(M.output) + ((uint64_t)row * samples + t0k);
 #pragma unroll
-  for (uint32_t h = 0; h < 2u; h++) {
-    const uint32_t k = lane + 64u * h;
-    if (tile * 128u + k < samples) {
+  for (uint32_t hh = 0; hh < 2u; hh++) {
+    const uint32_t k = lane + 64u * hh;
+    if (t0k + k < samples) {
       const unsigned long long key = L.key[k];
       if (key != kNoKey && key < slot[k]) atomicMin(slot + k, key);
     }
@@ -2053,10 +2050,11 @@ hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t sample
                      dim3(kBlock), 0, stream, params, dim, samples, sp, lo, delta, codes);
   if (coarse != nullptr) {
     if (sp % 8u != 0u) return hipErrorInvalidValue;
-    uint64_t cblocks = (total / 8u + kBlock - 1) / kBlock;
+    const uint32_t rb = coarse_row_bytes(sp);
+    uint64_t cblocks = ((uint64_t)dim * rb + kBlock - 1) / kBlock;
     if (cblocks > 65536u * 4u) cblocks = 65536u * 4u;
     hipLaunchKernelGGL(build_coarse_kernel, dim3((unsigned)std::max<uint64_t>(1, cblocks)),
-                       dim3(kBlock), 0, stream, codes, total / 8u, coarse);
+                       dim3(kBlock), 0, stream, codes, dim, sp, rb, coarse);
   }
   return hipGetLastError();
 }

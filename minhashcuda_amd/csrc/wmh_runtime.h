@@ -173,8 +173,8 @@ struct MinhashCudaGenerator_ {
   bool fast_div = false;
   // byte-coded bound table of the skip bound ([dim][sp] bytes, wmh_bound.h;
   // null when MHCUDA_SKIP_BOUND=0 or it could not be allocated) followed in
-  // the same allocation by its coarse codes (`coarse`, [dim][sp / 8] bytes,
-  // null unless sp % 8 == 0), with the scale and offset of its thresholds,
+  // the same allocation by its coarse codes (`coarse`, [dim][coarse_row_bytes]
+  // bytes, null unless sp % 8 == 0), with the scale and offset of its thresholds,
   // `bounded` while it matches the parameters and they pass the bound's range
   // check, and `skip` the user's switch (mhcuda_set_skip)
   uint8_t *bounds = nullptr;
