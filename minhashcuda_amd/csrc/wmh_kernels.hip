@@ -1114,6 +1114,11 @@ struct PieceLds {
   uint32_t npair, nitem;
 };
 
+// Base0: the gathers address the table from column 0 with 32-bit offsets
+// (the whole table below 4 GiB: no column is out of reach, whatever the row's
+// order), else from the stripe's first column (a piece holding a column out
+// of reach then returns false before writing anything).
+template <bool Base0>
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
                                                    const uint8_t *__restrict__ codes,
                                                    const uint8_t *__restrict__ coarse,
@@ -1126,7 +1131,7 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
   __shared__ PieceLds piece_lds[kWavesPerBlock];
   PieceLds &L = piece_lds[threadIdx.x >> 6];
   const uint32_t lane = lane_id();
-  const uint32_t col0 = uniform(col0_in);
+  const uint32_t col0 = Base0 ? 0u : uniform(col0_in);
   const uint32_t t0k = tile * 128u;  // the tile's first sample
   const uint32_t block_bytes = 12u * sp;  // the code table holds sp bytes per column
   const uint32_t s_b = uniform(sp * 4u), s_c = uniform(sp * 8u);  // beta, ln_c arrays
@@ -1171,9 +1176,13 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
   // samples 2l and 2l + 1 (the fine test), tc[i] that of the largest T' of
   // groups 2i and 2i + 1 (the coarse test, wave-uniform).
   uint32_t tc[8];
+  // (a sample without a minimum yet: FLT_MAX, whose T' skips nothing)
+  auto thr_of = [&](unsigned long long key) {
+    return wmh_code_threshold(key == kNoKey ? FLT_MAX : key_value(key), code_scale, code_t0);
+  };
   auto refresh = [&]() {
-    const uint32_t ta = wmh_code_threshold(key_value(L.key[2u * lane]), code_scale, code_t0);
-    const uint32_t tb = wmh_code_threshold(key_value(L.key[2u * lane + 1u]), code_scale, code_t0);
+    const uint32_t ta = thr_of(L.key[2u * lane]);
+    const uint32_t tb = thr_of(L.key[2u * lane + 1u]);
     L.tq[lane] = (0x4800u - ta) | ((0x4800u - tb) << 16);
     uint32_t m = max(ta, tb);
     m = max(m, (uint32_t)__shfl_xor((int)m, 1));
@@ -1219,7 +1228,7 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
     // a table of 4 GiB or more, or a stripe starting at dim, whose lanes
     // without a column point at column 0) is left to hash_piece_pairs,
     // before anything of it is written
-    if (__ballot(c - col0 > span) != 0ull) return false;
+    if (!Base0 && __ballot(c - col0 > span) != 0ull) return false;
     const bool fast = fast_div && __ballot(lw == __builtin_inff()) == 0ull;
     const uint32_t rel = c - col0;
     const uint32_t o = rel * block_bytes, ob = rel * sp;
@@ -1356,11 +1365,21 @@ __device__ __forceinline__ uint32_t find_matrix(const MatrixDesc *__restrict__ m
   return uniform(lo);
 }
 
+// Work of a persistent launch: every queue, with stripe pieces through the
+// skip bound or two nonzeros per step (kAllWork); every queue with every
+// stripe piece through the skip bound from column 0 (kBoundedPieces: without
+// the paired pieces' code the 2-samples kernel needs 98 VGPRs instead of 128,
+// 5 waves per SIMD, which the LDS of the bounded pieces allows); or the items
+// of the unstriped rows alone (kItemsOnly).
+enum PersistentMode { kAllWork = 0, kBoundedPieces = 1, kItemsOnly = 2 };
+constexpr int kPieceWavesPerCu = 20;
+
 // 4 waves per SIMD (128 VGPRs) at 1 and 2 samples per lane: left alone the
 // compiler takes 167 VGPRs for the 2-samples kernel and drops to 3 waves; the
 // 4-samples kernel (131 VGPRs) keeps 3.
-template <int V>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(V == 4 ? 3 : 4)))
+template <int V, int Mode>
+__global__ __launch_bounds__(kBlock) __attribute__((
+    amdgpu_waves_per_eu(Mode == kBoundedPieces ? 5 : (V == 4 ? 3 : 4))))
 void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
@@ -1413,7 +1432,8 @@ void wmh_persistent_kernel(
                                    : (step == 1 ? (items_first ? home : kMaxXcds)
                                                 : (home + step - 1) % Q);
     const uint32_t nk = qid < Q && qid < K ? (K - qid + Q - 1) / Q : 0u;
-    const uint64_t qtotal = qid == kMaxXcds ? item_units : per_stripe * nk;
+    uint64_t qtotal = qid == kMaxXcds ? item_units : per_stripe * nk;
+    if (Mode == kItemsOnly && qid < kMaxXcds) qtotal = 0;
     if (qtotal == 0) continue;
     for (;;) {
       uint64_t u = 0;
@@ -1465,7 +1485,19 @@ void wmh_persistent_kernel(
           row = ~0u;
         }
         const MatrixDesc M = mats[m];
-        if constexpr (V == 2) {
+        if (V == 2 && Mode == kBoundedPieces && piece) {
+          // every piece through the skip bound (its row's minima from heavy
+          // seeds; a sample without one is not skipped)
+          const uint32_t kq = tile * 128u + 8u * (lane & 15u);
+          const unsigned long long *slot = reinterpret_cast<const unsigned long long *>(M.output) +
+                                           ((uint64_t)row * samples + (kq < samples ? kq : 0u));
+          unsigned long long skey[8];
+#pragma unroll
+          for (int v = 0; v < 8; v++) skey[v] = slot[v];
+          hash_piece_bounded<true>(params, codes, coarse, code_scale, code_t0, dim, sp, samples,
+                                   tile, M, a, b, row, 0u, fast_div != 0u, skey);
+          continue;
+        } else if constexpr (V == 2 && Mode != kBoundedPieces) {
           if (piece) {
             const uint64_t col0 = (uint64_t)k * stripes.width;
             const uint32_t c0 = col0 < dim ? (uint32_t)col0 : dim;
@@ -1485,8 +1517,9 @@ void wmh_persistent_kernel(
                 none |= kq < samples && skey[v] == kNoKey;
               }
               if (__ballot(none) == 0ull &&
-                  hash_piece_bounded(params, codes, coarse, code_scale, code_t0, dim, sp,
-                                     samples, tile, M, a, b, row, c0, fast_div != 0u, skey))
+                  hash_piece_bounded<false>(params, codes, coarse, code_scale, code_t0, dim,
+                                            sp, samples, tile, M, a, b, row, c0,
+                                            fast_div != 0u, skey))
                 continue;
             }
             hash_piece_pairs(params, dim, sp, samples, tile, M, a, b, row, c0, fast_div != 0u);
@@ -2107,8 +2140,13 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   hipLaunchKernelGGL(batch_rows_kernel<kInit>, dim3(row_grid), dim3(kBlock), 0, stream,
                      params, samples, sp, mats, count, total_chunks, ctl, stripes);
   // heavy seeds where the stripe pieces use the skip bound
-  if (tuning.seeds && codes != nullptr && table.coarse != nullptr && samples % 8u == 0u &&
-      spl == 2 && stripes.stripes > 1)
+  const bool seeded = tuning.seeds && codes != nullptr && table.coarse != nullptr &&
+                      samples % 8u == 0u && spl == 2 && stripes.stripes > 1;
+  // With seeds and a table below 4 GiB, every stripe piece takes the skip
+  // bound from column 0 and the kernel leaves out the paired pieces (5 waves
+  // per SIMD instead of 4).
+  const bool split = seeded && (uint64_t)dim * 12u * sp < (1ull << 32);
+  if (seeded)
     hipLaunchKernelGGL(batch_seed_kernel,
                        dim3((unsigned)std::max(1, num_cus * kSeedBlocksPerCu)),
                        dim3(kBlock), 0, stream, params, dim, samples, sp, mats, ctl, stripes,
@@ -2117,17 +2155,25 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
   if (ev_begin) (void)hipEventRecord(ev_begin, stream);
   switch (spl) {
     case 1:
-      hipLaunchKernelGGL(wmh_persistent_kernel<1>, dim3(grid), dim3(kBlock), 0, stream,
+      hipLaunchKernelGGL((wmh_persistent_kernel<1, kAllWork>), dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
                          (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
       break;
     case 2:
-      hipLaunchKernelGGL(wmh_persistent_kernel<2>, dim3(grid), dim3(kBlock), 0, stream,
-                         params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
+      if (split) {
+        hipLaunchKernelGGL((wmh_persistent_kernel<2, kBoundedPieces>),
+                           dim3((unsigned)std::max(1, num_cus * kPieceWavesPerCu / kWavesPerBlock)),
+                           dim3(kBlock), 0, stream, params, dim, samples, sp, mats, count, ctl,
+                           stripes, (uint32_t)fast_div, codes, table.scale, table.t0,
+                           table.coarse);
+      } else {
+        hipLaunchKernelGGL((wmh_persistent_kernel<2, kAllWork>), dim3(grid), dim3(kBlock), 0,
+                           stream, params, dim, samples, sp, mats, count, ctl, stripes,
+                           (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
+      }
       break;
     default:
-      hipLaunchKernelGGL(wmh_persistent_kernel<4>, dim3(grid), dim3(kBlock), 0, stream,
+      hipLaunchKernelGGL((wmh_persistent_kernel<4, kAllWork>), dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
                          (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
       break;

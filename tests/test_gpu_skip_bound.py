@@ -250,6 +250,17 @@ def test_heavy_seeds_equal_the_oracle(libmhcuda, capi, gpu_available, samples):
             assert capi.mhcuda_set_tuning(g, 2, seg, 0) == 0
             got = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
             assert np.array_equal(got, ref), seg
+        # the same rows with their nonzeros in random order (the stripe pieces
+        # then gather from anywhere in the table)
+        rng2 = np.random.default_rng(samples + 6)
+        mu = m.copy()
+        for r in range(mu.shape[0]):
+            s0, e0 = mu.indptr[r], mu.indptr[r + 1]
+            perm = rng2.permutation(e0 - s0)
+            mu.indices[s0:e0] = mu.indices[s0:e0][perm]
+            mu.data[s0:e0] = mu.data[s0:e0][perm]
+        got_u = libmhcuda.minhash_cuda_calc_batch(gen, [mu], kernel=PERSISTENT)[0]
+        assert np.array_equal(got_u, oracle.calc(*params, mu.data, mu.indices, mu.indptr))
         assert capi.mhcuda_set_seeds(g, 0) == 0
         off = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
         assert np.array_equal(off, ref)
