@@ -216,9 +216,9 @@ def test_heavy_seeds_equal_the_oracle(libmhcuda, capi, gpu_available, samples):
     without them and as the oracle, for one and two 128-sample tiles and part
     tiles, with special weights, columns >= dim, a striped row of zeros only
     (no seed has a finite hash), a striped row whose every nonzero has a
-    column >= dim, and duplicate (column, weight) pairs of the heaviest weight
-    in one lane and in neighbouring lanes (equal hashes at two positions: the
-    earlier one wins)."""
+    column >= dim, duplicate (column, weight) pairs of the heaviest weight in
+    one lane and in neighbouring lanes (equal hashes at two positions: the
+    earlier one wins), and the same rows in random column order."""
     print("samples", samples, flush=True)
     dim = 1 << 15
     rng = np.random.default_rng(samples + 3)
