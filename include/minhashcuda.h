@@ -178,7 +178,8 @@ MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib
  * either way.  It needs tables with every r in [2^-60, 2^60], beta in
  * [0, 1) and |ln_c| <= 2^60 whose every |ln_c - r| stays within the fp16 range
  * (generated tables always qualify; assigned or loaded ones are checked),
- * and dim * roundup(samples, 4) bytes of device memory (allocated by
+ * and dim * roundup(samples, 4) bytes of device memory (plus dim * roundup(samples
+ * / 8, 16) bytes of coarse codes when samples % 8 == 0; allocated by
  * mhcuda_init; MHCUDA_SKIP_BOUND=0 in the environment skips that); the
  * stripe pieces also need a sample count divisible by 8.
  * mhcuda_set_skip(0) turns it off; mhcuda_get_skip reports the switch and
