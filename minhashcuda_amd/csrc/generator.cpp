@@ -83,6 +83,7 @@ MinhashCudaGenerator_::~MinhashCudaGenerator_() {
   if (d2h_stream) (void)hipStreamDestroy(d2h_stream);
   if (params) (void)hipFree(params);
   if (bounds) (void)hipFree(bounds);
+  if (items) (void)hipFree(items);
 }
 
 namespace {
@@ -302,6 +303,9 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen) {
       HIPCH(launch_build_codes(gen->params, gen->dim, gen->samples, lo, delta, gen->bounds,
                                gen->coarse, gen->stream),
             mhcudaRuntimeError);
+      if (gen->items)
+        HIPCH(launch_build_items(gen->params, gen->dim, gen->samples, gen->items, gen->stream),
+              mhcudaRuntimeError);
       HIPCH(hipStreamSynchronize(gen->stream), mhcudaRuntimeError);
       gen->code_scale = 1.0f / delta;
       gen->code_t0 = 2049.0f - lo / delta;
@@ -475,6 +479,16 @@ MinhashCudaGenerator *mhcuda_init(uint32_t dim, uint16_t samples, uint32_t seed,
       INFO("no memory for the bound table: hashing without the skip bound\n");
     } else if (gen->sp % 8u == 0u) {
       gen->coarse = gen->bounds + wmh::coarse_offset(dim, samples);
+      // the item table (4/3 of the parameter bytes) where 32-bit offsets reach it
+      const char *items_env = std::getenv("MHCUDA_ITEM_TABLE");
+      if (!(items_env && std::strcmp(items_env, "0") == 0) &&
+          wmh::item_table_fits(dim, samples) &&
+          hipMalloc((void **)&gen->items, (size_t)wmh::item_table_bytes(dim, samples)) !=
+              hipSuccess) {
+        (void)hipGetLastError();
+        gen->items = nullptr;
+        INFO("no memory for the item table: the bounded pieces read the parameter table\n");
+      }
     }
   }
   if (deferred) {

@@ -145,6 +145,8 @@ struct CodeTable {
   float scale;
   float t0;
   const uint8_t *coarse = nullptr;
+  // the item table (launch_build_items) or null
+  const float *items = nullptr;
 };
 
 // Bytes of a column's coarse codes: one per 8-sample group, padded to a
@@ -178,6 +180,19 @@ hipError_t launch_bound_range(const float *params, uint32_t dim, uint32_t sample
 // consecutive codes (padding bytes: 255).
 hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t samples, float lo,
                               float delta, uint8_t *codes, uint8_t *coarse, hipStream_t stream);
+
+// The item table of the bounded stripe pieces: items [dim][sp][4] holds the
+// (r, beta, ln_c, 0) of every (column, sample) side by side (one memory line
+// per exact evaluation instead of three).  Used only where its bytes stay
+// below 2^32 (item_table_fits), which the 32-bit offsets of the gathers need.
+inline uint64_t item_table_bytes(uint32_t dim, uint32_t samples) {
+  return (uint64_t)dim * padded_samples(samples) * 16u;
+}
+inline bool item_table_fits(uint32_t dim, uint32_t samples) {
+  return item_table_bytes(dim, samples) <= (1ull << 32);
+}
+hipError_t launch_build_items(const float *params, uint32_t dim, uint32_t samples, float *items,
+                              hipStream_t stream);
 
 // Scatter one [samples][ncols] FP32 table (rows of pitch ncols) into
 // params[(j0 + jj) * 3 * sp + which * sp + k].

@@ -267,6 +267,22 @@ __global__ __launch_bounds__(kBlock) void build_coarse_kernel(const uint8_t *__r
   }
 }
 
+// The item table: (r, beta, ln_c, 0) of every (column, sample) of the
+// column-block parameter table side by side, [dim][sp] entries of 16 bytes,
+// so that an exact evaluation reads one memory line instead of three.
+__global__ __launch_bounds__(kBlock) void build_items_kernel(const float *__restrict__ params,
+                                                             uint32_t dim, uint32_t sp,
+                                                             float4 *__restrict__ items) {
+  const uint64_t total = (uint64_t)dim * sp;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t j = i / sp;
+    const uint32_t k = (uint32_t)(i - j * sp);
+    const float *blk = params + j * 3u * sp;
+    items[i] = make_float4(blk[k], blk[sp + k], blk[2u * sp + k], 0.0f);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The hashing core.
 
@@ -1117,11 +1133,14 @@ struct PieceLds {
 // Base0: the gathers address the table from column 0 with 32-bit offsets
 // (the whole table below 4 GiB: no column is out of reach, whatever the row's
 // order), else from the stripe's first column (a piece holding a column out
-// of reach then returns false before writing anything).
+// of reach then returns false before writing anything).  `items`: the item
+// table (build_items_kernel) when Base0 and it lies below 4 GiB, else null
+// (the items read the column-block table).
 template <bool Base0>
 __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ params,
                                                    const uint8_t *__restrict__ codes,
                                                    const uint8_t *__restrict__ coarse,
+                                                   const float4 *__restrict__ items,
                                                    float code_scale, float code_t0,
                                                    uint32_t dim, uint32_t sp, uint32_t samples,
                                                    uint32_t tile, const MatrixDesc &M, uint32_t a,
@@ -1156,6 +1175,11 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
                                          uniform((uint32_t)cbb));
   const __amdgpu_buffer_rsrc_t rc =
       __builtin_amdgcn_make_buffer_rsrc(cbase, (short)0, (int)0xffffffff, kRawBufferWord3);
+  const uint64_t ib = reinterpret_cast<uint64_t>(items);
+  char *ibase = reinterpret_cast<char *>(((uint64_t)uniform((uint32_t)(ib >> 32)) << 32) |
+                                         uniform((uint32_t)ib));
+  const __amdgpu_buffer_rsrc_t ri =
+      __builtin_amdgcn_make_buffer_rsrc(ibase, (short)0, (int)0xffffffff, kRawBufferWord3);
   const uint32_t *__restrict__ cols = M.cols - M.base;
   const float *__restrict__ weights = M.weights - M.base;
   // groups of the tile that hold samples (samples % 8 == 0: whole groups)
@@ -1310,13 +1334,24 @@ __device__ __forceinline__ bool hash_piece_bounded(const float *__restrict__ par
           const uint32_t isrc = it & 63u;
           const uint32_t k = it >> 6;  // the sample's place in the tile
           const float lws = __uint_as_float(bperm_u32(__float_as_uint(lw), isrc));
-          const uint32_t off = (bperm_u32(o, isrc) + 4u * (t0k + k)) & (ivalid ? ~0u : 0u);
-          const float r =
-              __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0u, 0));
-          const float beta =
-              __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_b, 0));
-          const float lc =
-              __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_c, 0));
+          float r, beta, lc;
+          if (Base0 && items != nullptr) {
+            // the item's (r, beta, ln_c) in one 16-byte load (below 2^32:
+            // the item table lies below 4 GiB)
+            const uint32_t off =
+                ((bperm_u32(ob, isrc) + t0k + k) << 4) & (ivalid ? ~0u : 0u);
+            const float4 q = __builtin_bit_cast(
+                float4, __builtin_amdgcn_raw_buffer_load_b128(ri, off, 0u, 0));
+            r = q.x;
+            beta = q.y;
+            lc = q.z;
+          } else {
+            const uint32_t off = (bperm_u32(o, isrc) + 4u * (t0k + k)) & (ivalid ? ~0u : 0u);
+            r = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0u, 0));
+            beta =
+                __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_b, 0));
+            lc = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, s_c, 0));
+          }
           float ln_a;
           {
             WMH_NO_CONTRACT
@@ -1384,7 +1419,7 @@ void wmh_persistent_kernel(
     const float *__restrict__ params, uint32_t dim, uint32_t samples, uint32_t sp,
     const MatrixDesc *__restrict__ mats, uint32_t count, BatchCtl *__restrict__ ctl,
     StripePlan stripes, uint32_t fast_div, const uint8_t *__restrict__ codes, float code_scale,
-    float code_t0, const uint8_t *__restrict__ coarse) {
+    float code_t0, const uint8_t *__restrict__ coarse, const float4 *__restrict__ items) {
   const uint32_t lane = lane_id();
   if (uniform(ctl->invalid)) return;  // malformed indptr: hash nothing
   const uint32_t seg = uniform(ctl->seg);
@@ -1494,8 +1529,8 @@ void wmh_persistent_kernel(
           unsigned long long skey[8];
 #pragma unroll
           for (int v = 0; v < 8; v++) skey[v] = slot[v];
-          hash_piece_bounded<true>(params, codes, coarse, code_scale, code_t0, dim, sp, samples,
-                                   tile, M, a, b, row, 0u, fast_div != 0u, skey);
+          hash_piece_bounded<true>(params, codes, coarse, items, code_scale, code_t0, dim, sp,
+                                   samples, tile, M, a, b, row, 0u, fast_div != 0u, skey);
           continue;
         } else if constexpr (V == 2 && Mode != kBoundedPieces) {
           if (piece) {
@@ -1517,8 +1552,8 @@ void wmh_persistent_kernel(
                 none |= kq < samples && skey[v] == kNoKey;
               }
               if (__ballot(none) == 0ull &&
-                  hash_piece_bounded<false>(params, codes, coarse, code_scale, code_t0, dim,
-                                            sp, samples, tile, M, a, b, row, c0,
+                  hash_piece_bounded<false>(params, codes, coarse, nullptr, code_scale, code_t0,
+                                            dim, sp, samples, tile, M, a, b, row, c0,
                                             fast_div != 0u, skey))
                 continue;
             }
@@ -2092,6 +2127,16 @@ hipError_t launch_build_codes(const float *params, uint32_t dim, uint32_t sample
   return hipGetLastError();
 }
 
+hipError_t launch_build_items(const float *params, uint32_t dim, uint32_t samples, float *items,
+                              hipStream_t stream) {
+  const uint32_t sp = padded_samples(samples);
+  uint64_t blocks = ((uint64_t)dim * sp + kBlock - 1) / kBlock;
+  if (blocks > 65536u * 4u) blocks = 65536u * 4u;
+  hipLaunchKernelGGL(build_items_kernel, dim3((unsigned)std::max<uint64_t>(1, blocks)),
+                     dim3(kBlock), 0, stream, params, dim, sp, reinterpret_cast<float4 *>(items));
+  return hipGetLastError();
+}
+
 hipError_t launch_histogram(const MatrixDesc *mats, uint32_t count, uint32_t total_chunks,
                             unsigned long long *hist, hipStream_t stream) {
   const unsigned grid = std::min<unsigned>(grid_for(total_chunks), 1024u);
@@ -2157,7 +2202,7 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
     case 1:
       hipLaunchKernelGGL((wmh_persistent_kernel<1, kAllWork>), dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
+                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse, nullptr);
       break;
     case 2:
       if (split) {
@@ -2165,17 +2210,17 @@ hipError_t launch_persistent(const float *params, uint32_t dim, uint32_t samples
                            dim3((unsigned)std::max(1, num_cus * kPieceWavesPerCu / kWavesPerBlock)),
                            dim3(kBlock), 0, stream, params, dim, samples, sp, mats, count, ctl,
                            stripes, (uint32_t)fast_div, codes, table.scale, table.t0,
-                           table.coarse);
+                           table.coarse, reinterpret_cast<const float4 *>(table.items));
       } else {
         hipLaunchKernelGGL((wmh_persistent_kernel<2, kAllWork>), dim3(grid), dim3(kBlock), 0,
                            stream, params, dim, samples, sp, mats, count, ctl, stripes,
-                           (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
+                           (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse, nullptr);
       }
       break;
     default:
       hipLaunchKernelGGL((wmh_persistent_kernel<4, kAllWork>), dim3(grid), dim3(kBlock), 0, stream,
                          params, dim, samples, sp, mats, count, ctl, stripes,
-                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse);
+                         (uint32_t)fast_div, codes, table.scale, table.t0, table.coarse, nullptr);
       break;
   }
   if (ev_end) (void)hipEventRecord(ev_end, stream);

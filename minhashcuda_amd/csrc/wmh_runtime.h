@@ -179,6 +179,10 @@ struct MinhashCudaGenerator_ {
   // check, and `skip` the user's switch (mhcuda_set_skip)
   uint8_t *bounds = nullptr;
   uint8_t *coarse = nullptr;
+  // the item table of the bounded stripe pieces (wmh::launch_build_items;
+  // null when MHCUDA_ITEM_TABLE=0, without coarse codes, at 4 GiB or more, or
+  // when it could not be allocated), rebuilt with the bound table
+  float *items = nullptr;
   float code_scale = 0.0f, code_t0 = 0.0f;
   bool bounded = false;
   bool skip = true;
@@ -255,11 +259,12 @@ MHCUDAResult refresh_division_mode(MinhashCudaGenerator *gen);
 // off or unusable for these tables.
 inline CodeTable launch_bounds_of(const MinhashCudaGenerator *gen) {
   if (!(gen->skip && gen->bounded)) return CodeTable{nullptr, 0.0f, 0.0f};
-  return CodeTable{gen->bounds, gen->code_scale, gen->code_t0, gen->coarse};
+  return CodeTable{gen->bounds, gen->code_scale, gen->code_t0, gen->coarse, gen->items};
 }
 
 inline bool same_table(const CodeTable &x, const CodeTable &y) {
-  return x.codes == y.codes && x.scale == y.scale && x.t0 == y.t0;
+  return x.codes == y.codes && x.scale == y.scale && x.t0 == y.t0 && x.coarse == y.coarse &&
+         x.items == y.items;
 }
 
 // Device-pointer calls given no stream run on gen->stream, which does not

@@ -210,16 +210,22 @@ def test_row_walk_bound_with_stripes_and_short_rows(libmhcuda, capi, gpu_availab
         libmhcuda.minhash_cuda_fini(gen)
 
 
+@pytest.mark.parametrize("item_table", ["1", "0"])
 @pytest.mark.parametrize("samples", [128, 256, 64, 200])
-def test_heavy_seeds_equal_the_oracle(libmhcuda, capi, gpu_available, samples):
+def test_heavy_seeds_equal_the_oracle(libmhcuda, capi, gpu_available, monkeypatch, samples,
+                                      item_table):
     """Heavy seeds of the striped rows (mhcuda_set_seeds): the same hashes as
     without them and as the oracle, for one and two 128-sample tiles and part
     tiles, with special weights, columns >= dim, a striped row of zeros only
     (no seed has a finite hash), a striped row whose every nonzero has a
     column >= dim, duplicate (column, weight) pairs of the heaviest weight in
     one lane and in neighbouring lanes (equal hashes at two positions: the
-    earlier one wins), and the same rows in random column order."""
-    print("samples", samples, flush=True)
+    earlier one wins), and the same rows in random column order; the bounded
+    pieces reading the interleaved item table and (MHCUDA_ITEM_TABLE=0) the
+    column-block parameter table, and the item table rebuilt when new
+    parameters are assigned."""
+    print("samples", samples, "item table", item_table, flush=True)
+    monkeypatch.setenv("MHCUDA_ITEM_TABLE", item_table)
     dim = 1 << 15
     rng = np.random.default_rng(samples + 3)
     nnz = np.concatenate([rng.integers(1100, 9000, 10), rng.integers(1, 300, 20),
@@ -264,5 +270,17 @@ def test_heavy_seeds_equal_the_oracle(libmhcuda, capi, gpu_available, samples):
         assert capi.mhcuda_set_seeds(g, 0) == 0
         off = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
         assert np.array_equal(off, ref)
+        # other parameters (those of another seed): the bound and item tables
+        # are rebuilt from them
+        other = libmhcuda.minhash_cuda_init(dim, samples, seed=samples + 7, devices=1)
+        try:
+            params2 = libmhcuda.minhash_cuda_retrieve_vars(other)
+        finally:
+            libmhcuda.minhash_cuda_fini(other)
+        libmhcuda.minhash_cuda_assign_vars(gen, *params2)
+        assert skip_state(capi, g) == (1, 1)
+        assert capi.mhcuda_set_seeds(g, 1) == 0
+        got2 = libmhcuda.minhash_cuda_calc_batch(gen, [m], kernel=PERSISTENT)[0]
+        assert np.array_equal(got2, oracle.calc(*params2, m.data, m.indices, m.indptr))
     finally:
         libmhcuda.minhash_cuda_fini(gen)
