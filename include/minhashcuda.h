@@ -179,9 +179,12 @@ MHCUDAResult mhcuda_get_stripes(const MinhashCudaGenerator *gen, int *stripe_mib
  * [0, 1) and |ln_c| <= 2^60 whose every |ln_c - r| stays within the fp16 range
  * (generated tables always qualify; assigned or loaded ones are checked),
  * and dim * roundup(samples, 4) bytes of device memory (plus dim * roundup(samples
- * / 8, 16) bytes of coarse codes when samples % 8 == 0; allocated by
- * mhcuda_init; MHCUDA_SKIP_BOUND=0 in the environment skips that); the
- * stripe pieces also need a sample count divisible by 8.
+ * / 8, 16) bytes of coarse codes when samples % 8 == 0, and then, while
+ * 16 * dim * roundup(samples, 4) bytes stay within 4 GiB, that many bytes of
+ * an item table of interleaved parameters; allocated by mhcuda_init;
+ * MHCUDA_SKIP_BOUND=0 in the environment skips all of them,
+ * MHCUDA_ITEM_TABLE=0 the item table alone); the stripe pieces also need a
+ * sample count divisible by 8.
  * mhcuda_set_skip(0) turns it off; mhcuda_get_skip reports the switch and
  * whether the bound is in use for the current tables. */
 MHCUDAResult mhcuda_set_skip(MinhashCudaGenerator *gen, int enable);
